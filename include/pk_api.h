@@ -1,0 +1,154 @@
+/*
+ * pk_api.h -- C ABI of the MI355X (gfx950) power-spectrum hot path.
+ *
+ * Path: particles -> CIC/TSC mass assignment -> density contrast -> 3D R2C FFT ->
+ *       window deconvolution -> |delta_k|^2 k-shell binning with Legendre l = 0, 2, 4
+ *       weights -> P_l(k) (+ shot-noise subtraction).
+ *
+ * The reference (dforero0896/jax-powspec) is pure Python/JAX and has no FFI; each entry
+ * point below replaces the reference function named in its comment.  The Python drop-in
+ * (jax_powspec_amd.mas / jax_powspec_amd.correlations) binds these symbols with ctypes.
+ *
+ * Conventions
+ *   - every function returns int status: PK_OK (0) or a negative PK_E_* code; the message
+ *     of the last failure on the calling thread is returned by pk_last_error().
+ *   - all array pointers are DEVICE pointers (HBM) unless the comment says "host";
+ *     the library never frees caller memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream); every launch
+ *     is stream-ordered and asynchronous; nothing synchronises the host except
+ *     pk_timing_read().
+ *   - a pk_plan owns only small device tables (twiddles, window tables) for one mesh side n
+ *     on the device that was current at creation; scratch memory is caller-provided
+ *     (query the size with the *_workspace_size functions).
+ *   - a plan may be used from one stream at a time; use one plan per concurrent stream.
+ *
+ * Layouts
+ *   - mesh: float32 [n][n][n], C order (x slowest, z fastest) -- the reference's delta.
+ *   - spectrum: complex64 (float2) [n/2+1][n][n] indexed [kz][kx][ky] (ky fastest).
+ *     Entry (kz, i, j) equals numpy.fft.rfftn(mesh)[i, j, kz].
+ *   - bin thresholds: int32 thr[nb+1]; bin b holds modes with thr[b] <= m < thr[b+1] where
+ *     m = kx^2 + ky^2 + kz^2 (integer wavenumbers), except that thr[nb] is the first m above
+ *     the closed last edge.  jstart: int32[jmax+1], jstart[j] = #{b < nb : thr[b] <= j*j} - 1.
+ *     Both are produced on the host from the FP64 edges (jax_powspec_amd.binning).
+ */
+#ifndef PK_API_H
+#define PK_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_API_VERSION 1
+
+#define PK_OK 0
+#define PK_E_INVALID (-1)      /* bad argument (size, stride, unsupported n, ...) */
+#define PK_E_OOM (-2)          /* device allocation failed (plan tables only) */
+#define PK_E_LAUNCH (-3)       /* a HIP call or kernel launch failed */
+#define PK_E_UNSUPPORTED (-4)  /* valid request this build does not implement */
+#define PK_E_WORKSPACE (-5)    /* workspace smaller than the queried size */
+
+/* mass-assignment stencils */
+#define PK_STENCIL_CIC_REF 0   /* reference stencil of src/mas.py:142-151 (incl. the :149 corner) */
+#define PK_STENCIL_CIC 1       /* textbook mass-conserving periodic CIC */
+#define PK_STENCIL_TSC 2       /* textbook periodic TSC */
+
+/* +1-neighbour wrap rules of the reference stencil (ignored by CIC/TSC, which are periodic) */
+#define PK_WRAP_VEC_TRUE 0     /* src/mas.py:119-122 (cic_mas_vec, wrap=True) */
+#define PK_WRAP_VEC_FALSE 1    /* src/mas.py:123-134 (cic_mas_vec, wrap=False) */
+#define PK_WRAP_SCAN_TRUE 2    /* src/mas.py:20-28   (cic_mas, wrap=True) */
+#define PK_WRAP_SCAN_FALSE 3   /* src/mas.py:30-38   (cic_mas, wrap=False) */
+
+/* flags for pk_powspec_mesh / pk_estimate */
+#define PK_FLAG_ZERO_DC 1      /* treat delta_k at k = 0 as exactly 0 (contrast was applied) */
+#define PK_FLAG_KMEAN 2        /* also accumulate sum sqrt(m) per bin (mode-averaged k) */
+
+/* stats buffer (device double[4]) written by pk_scatter / pk_estimate:
+ *   [0] deposited mass M, [1] n^3 / M (1 / mean density), [2] scratch, [3] scratch */
+#define PK_STATS_LEN 4
+
+typedef struct pk_plan pk_plan;
+
+int pk_version(void);
+const char* pk_last_error(void);
+
+/* --- plans ------------------------------------------------------------------------- */
+/* Supported n: even, 8 <= n <= 1024, n = 2^a or 3 * 2^a. */
+int pk_plan_create(int n, pk_plan** plan);
+int pk_plan_destroy(pk_plan* plan);
+int pk_supported_n(int n);   /* 1 if n is supported, else 0 */
+
+/* --- mass assignment ----------------------------------------------------------------
+ * Replaces cic_mas_vec (src/mas.py:88-153) and cic_mas (src/mas.py:5-87); CIC/TSC are
+ * extensions.  mesh_out = mesh_in (or zeros if mesh_in is NULL) + deposits; mesh_out may
+ * alias mesh_in.  Positions are element-strided (stride 3 reads an (N_p, 3) array in
+ * place); w may be NULL (unit weights).  tile[3] = (tx, ty, tz) cell-tile of the sorted
+ * LDS-privatised deposit; tile == NULL picks the default; tile[0] == 0 selects direct
+ * global atomics.  stats (nullable) receives the deposited mass (PK_STATS_LEN doubles).
+ * Index/fraction arithmetic follows the reference in float32: inv = 1 / ((float)box /
+ * (float)n), pos = (x - xmin) * inv.
+ */
+int pk_scatter_workspace_size(int n, int64_t np, int stencil, const int* tile, size_t* bytes);
+int pk_scatter(const float* x, int64_t sx, const float* y, int64_t sy, const float* z, int64_t sz,
+               const float* w, int64_t sw, int64_t np,
+               float xmin, float ymin, float zmin, double box_size, int n,
+               int stencil, int wrap_mode, const int* tile,
+               const float* mesh_in, float* mesh_out, double* stats,
+               void* workspace, size_t workspace_bytes, void* stream);
+
+/* --- R2C FFT ------------------------------------------------------------------------
+ * Replaces jnp.fft.rfftn(delta) of src/correlations.py:22 (unnormalised, e^{-i k.x}).
+ * If stats is non-NULL the contrast delta = rho * stats[1] - 1 (tests/lognormal.py:79-80)
+ * is applied while the mesh is read.  Writes the spectrum layout described above.
+ */
+int pk_r2c(pk_plan* plan, const float* mesh, const double* stats, void* spectrum, void* stream);
+
+/* --- P_l(k) from a mesh -------------------------------------------------------------
+ * Replaces powspec_vec (src/correlations.py:7-56) and powspec_vec_fundamental
+ * (src/correlations.py:60-117) together with the caller-side contrast and shot noise.
+ * Runs the z/y FFT passes into `spectrum` (caller buffer, (n/2+1) n^2 float2), then the
+ * x pass fused with deconvolution (x / sin x)^(2p) per axis, |delta_k|^2, Legendre
+ * weights and FP64 per-bin sums.  Outputs (device, nb entries; any may be NULL except
+ * pk): sums[nb][3] FP64 (S0, S2, S4), counts[nb] int64, pk[nb][3] float32
+ * = S_l / N * (2l+1) * (box/n^2)^3 with P_0 -= shot_noise (NaN for empty bins),
+ * nmodes[nb] float32, kmean[nb] FP64 (needs PK_FLAG_KMEAN; units of k_F).
+ */
+int pk_powspec_workspace_size(pk_plan* plan, int nb, size_t* bytes);
+int pk_powspec_mesh(pk_plan* plan, const float* mesh, const double* stats,
+                    const int32_t* thr, const int32_t* jstart, int nb, int window_p, int flags,
+                    double box_size, double shot_noise, void* spectrum,
+                    double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* --- whole estimate -----------------------------------------------------------------
+ * particles -> assignment (mesh_in = zeros) -> contrast folded into the FFT read ->
+ * P_l(k).  Workspace holds the mesh, the spectrum and all scratch.
+ */
+int pk_estimate_workspace_size(pk_plan* plan, int64_t np, int stencil, const int* tile, int nb,
+                               size_t* bytes);
+int pk_estimate(pk_plan* plan,
+                const float* x, int64_t sx, const float* y, int64_t sy, const float* z, int64_t sz,
+                const float* w, int64_t sw, int64_t np,
+                float xmin, float ymin, float zmin, double box_size,
+                int stencil, int wrap_mode, const int* tile,
+                const int32_t* thr, const int32_t* jstart, int nb, int window_p, int flags,
+                double shot_noise,
+                double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
+                double* stats, void* workspace, size_t workspace_bytes, void* stream);
+
+/* --- per-stage timing (bench / profiling; no reference counterpart) ----------------
+ * When enabled, each kernel stage is bracketed by HIP events on its own stream.
+ * pk_timing_read synchronises those events, returns total ms and launch counts per stage
+ * (arrays of PK_NUM_STAGES) and clears the record.
+ */
+#define PK_NUM_STAGES 12
+int pk_timing_enable(int on);
+int pk_timing_read(double* ms, int64_t* launches, int nstages);
+const char* pk_stage_name(int stage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PK_API_H */

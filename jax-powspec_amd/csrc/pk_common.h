@@ -1,0 +1,206 @@
+// Shared device helpers: complex arithmetic, small-radix DFTs and the LDS-resident
+// mixed-radix Stockham stages used by every FFT pass.  gfx950 (wave64) only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pk_api.h"
+
+namespace pk {
+
+// ------------------------------------------------------------------------------------
+// host-side error state (defined in pk_capi.hip)
+// ------------------------------------------------------------------------------------
+int set_error(int code, const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+
+enum Stage {
+  ST_SORT_COUNT = 0, ST_SCAN, ST_SORT_PLACE, ST_DEPOSIT, ST_ATOMIC, ST_MASS,
+  ST_ZPASS, ST_YPASS, ST_XPASS_BIN, ST_FINALIZE, ST_XPASS_PLAIN, ST_FILL
+};
+void timing_begin(int stage, hipStream_t s);
+void timing_end(int stage, hipStream_t s);
+
+struct StageTimer {
+  int stage; hipStream_t s;
+  StageTimer(int st, hipStream_t str) : stage(st), s(str) { timing_begin(stage, s); }
+  ~StageTimer() { timing_end(stage, s); }
+};
+
+// ------------------------------------------------------------------------------------
+// complex helpers
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * (-i)
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// forward DFTs (e^{-2 pi i r k / R}) in registers, natural-order output
+template <int R> struct DFT;
+
+template <> struct DFT<2> {
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <> struct DFT<3> {
+  static __device__ __forceinline__ void run(float2* v) {
+    const float s = 0.86602540378443864676f;  // sin(pi/3)
+    float2 t = cadd(v[1], v[2]);
+    float2 d = csub(v[1], v[2]);
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    v[0] = cadd(v[0], t);
+    // X1 = m - i s d ; X2 = m + i s d
+    v[1] = make_float2(m.x + s * d.y, m.y - s * d.x);
+    v[2] = make_float2(m.x - s * d.y, m.y + s * d.x);
+  }
+};
+
+template <> struct DFT<4> {
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]);
+    float2 t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]);
+    float2 t3 = cmul_mi(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <> struct DFT<8> {
+  static __device__ __forceinline__ void run(float2* v) {
+    const float c = 0.70710678118654752440f;  // sqrt(1/2)
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    DFT<4>::run(e);
+    DFT<4>::run(o);
+    // o[k] *= e^{-i pi k / 4}
+    float2 o1 = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
+    float2 o2 = cmul_mi(o[2]);
+    float2 o3 = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+// ------------------------------------------------------------------------------------
+// radix plan: as many 8s as possible, then 4, then 2, then 3
+// ------------------------------------------------------------------------------------
+__host__ __device__ constexpr int radix_next(int m) {
+  return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 1;
+}
+__host__ __device__ constexpr int radix_at(int N, int s) {
+  int m = N;
+  for (int st = 0; st < 16; ++st) {
+    int r = radix_next(m);
+    if (m == 1 || r == 1) return 1;
+    if (st == s) return r;
+    m /= r;
+  }
+  return 1;
+}
+__host__ __device__ constexpr bool fft_size_ok(int N) {
+  int m = N;
+  for (int st = 0; st < 16 && m > 1; ++st) {
+    int r = radix_next(m);
+    if (r == 1) return false;
+    m /= r;
+  }
+  return m == 1;
+}
+
+// ------------------------------------------------------------------------------------
+// One Stockham stage over C independent length-N transforms held in LDS.
+// Element (point p, column c) lives at buf[p * PS + c * CS].  The stage reads radix-R
+// butterflies at stride N/R, applies twiddles w^(r k) with k = j mod NS, runs the
+// R-point DFT and writes the outputs at (j / NS) * NS * R + k + r * NS (autosort).
+// All reads finish (barrier) before any write, so one buffer suffices.
+// tw: table of e^{-2 pi i q / (N * TWS)}, q < N * TWS (stride TWS selects length N).
+// COLFAST: consecutive threads take consecutive columns (else consecutive butterflies).
+// ------------------------------------------------------------------------------------
+template <int N, int R, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS>
+__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
+  constexpr int B = N / R;
+  constexpr int TASKS = B * C;
+  constexpr int PER = (TASKS + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  float2 v[PER][R];
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int task = tid + t * NT;
+    if ((TASKS % NT) == 0 || task < TASKS) {
+      const int j = COLFAST ? task / C : task % B;
+      const int c = COLFAST ? task % C : task / B;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[t][r] = buf[(j + r * B) * PS + c * CS];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int task = tid + t * NT;
+    if ((TASKS % NT) == 0 || task < TASKS) {
+      const int j = COLFAST ? task / C : task % B;
+      const int c = COLFAST ? task % C : task / B;
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], tw[(r * k * (N / (NS * R))) * TWS]);
+      }
+      DFT<R>::run(v[t]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * NS) * PS + c * CS] = v[t][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int S, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS>
+struct FFTStages {
+  static __device__ __forceinline__ void run(float2* buf, const float2* tw) {
+    constexpr int R = radix_at(N, S);
+    if constexpr (R > 1) {
+      fft_stage<N, R, NS, C, PS, CS, NT, COLFAST, TWS>(buf, tw);
+      FFTStages<N, S + 1, NS * R, C, PS, CS, NT, COLFAST, TWS>::run(buf, tw);
+    }
+  }
+};
+
+// forward length-N transforms of C columns in LDS (see fft_stage for the layout)
+template <int N, int C, int PS, int CS, int NT, bool COLFAST, int TWS>
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
+  static_assert(fft_size_ok(N), "unsupported FFT length");
+  FFTStages<N, 0, 1, C, PS, CS, NT, COLFAST, TWS>::run(buf, tw);
+}
+
+// ------------------------------------------------------------------------------------
+// misc
+// ------------------------------------------------------------------------------------
+__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+__host__ __device__ constexpr int pow2_floor(int v) { return v <= 1 ? 1 : 2 * pow2_floor(v / 2); }
+// largest power of two dividing v
+__host__ __device__ constexpr int pow2_div(int v) { return (v & 1) ? 1 : 2 * pow2_div(v / 2); }
+__host__ __device__ constexpr int isqrt_c(int v) {
+  int r = 0;
+  while ((r + 1) * (r + 1) <= v) ++r;
+  return r;
+}
+__host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+}  // namespace pk

@@ -1,0 +1,446 @@
+// 3D real-to-complex FFT passes and the fused x-pass + deconvolution + Legendre binning.
+//
+// Spectrum layout: float2 [n/2+1][n][n] indexed [kz][kx][ky] (ky fastest), so
+//   z pass : real rows along z (contiguous in the mesh) -> R2C via a length-n/2 complex
+//            FFT + split; writes [kz][x][y] with 16+ consecutive y per 128 B segment;
+//   y pass : contiguous rows along y, in place;
+//   x pass : columns along x at stride n, tiles of CX consecutive ky; fused with the
+//            binning so the final spectrum is never written back (pk_powspec_mesh),
+//            or written back in place (pk_r2c).
+// Every pass is a persistent grid (occupancy-sized) looping over tiles; each tile is
+// staged in LDS and transformed with the Stockham stages of pk_common.h.
+//
+// Reference semantics (src/correlations.py:7-56, :60-117):
+//   delta_k = rfftn(delta) (:22); correction prod_a (x_a / sin x_a)^p, x_a = pi k_a / n
+//   (:20-21, :32, :39); P2 = |delta_k|^2 (:40); mu = kz / k, LOS = z (:36-38);
+//   l = 0, 2, 4 weights (:45-47); every stored half-spectrum entry counts once (:48).
+#include "pk_common.h"
+#include "pk_internal.h"
+
+namespace pk {
+
+template <int N> struct FFTCfg {
+  static constexpr int NC = N / 2;
+  // z pass: CZ real rows (consecutive y) per tile
+  static constexpr int CZ = cmin(pow2_div(N), cmin(64, cmax(8, pow2_floor(4096 / NC))));
+  static constexpr int NTZ = cmin(1024, cmax(64, (CZ * NC / 16) / 64 * 64));
+  // y pass: CY rows per tile
+  static constexpr int CY = cmin(pow2_div(N), cmin(64, cmax(1, pow2_floor(4096 / N))));
+  static constexpr int NTY = cmin(1024, cmax(64, (CY * N / 16) / 64 * 64));
+  // x pass: CX columns (consecutive ky) per tile
+  static constexpr int CX = cmin(pow2_div(N), cmin(64, cmax(N >= 1024 ? 8 : 16, pow2_floor(4096 / N))));
+  static constexpr int NTX = cmin(1024, cmax(64, (CX * N / 16) / 64 * 64));
+};
+
+// ------------------------------------------------------------------------------------
+// z pass: real rows -> half spectrum rows, written transposed to [kz][x][y]
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTZ)
+k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2* __restrict__ twg,
+        const double* __restrict__ stats, double* __restrict__ dc_part) {
+  using Cf = FFTCfg<N>;
+  constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = N * YT;
+  __shared__ float2 buf[C * CS];
+  __shared__ float2 tw[N];
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  double dc = 0.0;  // FP64 sum of the transformed values = the k = 0 mode
+  float s = 1.0f, b = 0.0f;
+  if (stats != nullptr) {  // contrast delta = rho / mean - 1 (tests/lognormal.py:79-80)
+    s = (float)stats[1];
+    b = -1.0f;
+  }
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int x = tile / YT, yt = tile % YT;
+    const float2* src = reinterpret_cast<const float2*>(mesh + ((size_t)x * N + (size_t)yt * C) * N);
+    for (int i = tid; i < C * NC; i += NT) {
+      float2 v = src[i];
+      v.x = v.x * s + b;
+      v.y = v.y * s + b;
+      dc += (double)v.x + (double)v.y;
+      buf[(i / NC) * CS + (i % NC)] = v;
+    }
+    __syncthreads();
+    lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+    // split: X[k] = Fe + e^{-2 pi i k / N} Fo, Fe = (Z_k + Z*_{NC-k}) / 2, Fo = -i (Z_k - Z*_{NC-k}) / 2
+    for (int i = tid; i < (NC + 1) * C; i += NT) {
+      const int c = i % C, k = i / C;
+      const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
+      const float2 zr = buf[c * CS + (k == 0 ? 0 : NC - k)];
+      const float2 zc = make_float2(zr.x, -zr.y);
+      const float2 fe = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+      const float2 dd = csub(zk, zc);
+      const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
+      const float2 X = cadd(fe, cmul(tw[k], fo));
+      spec[((size_t)k * N + x) * N + (size_t)yt * C + c] = X;
+    }
+    __syncthreads();
+  }
+  if (dc_part != nullptr) {  // fixed-order tree: the k = 0 mode in FP64 (sum of all values)
+    red[tid] = dc;
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+      if (tid < off) red[tid] += red[tid + off];
+      __syncthreads();
+    }
+    if (tid == 0) dc_part[blockIdx.x] = red[0];
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// y pass: contiguous rows of N complex, in place
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTY)
+k_ypass(float2* __restrict__ spec, const float2* __restrict__ twg) {
+  using Cf = FFTCfg<N>;
+  constexpr int C = Cf::CY, NT = Cf::NTY, CS = N + 1;
+  constexpr int NTILES = (N / 2 + 1) * N / C;
+  __shared__ float2 buf[C * CS];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    float2* row = spec + (size_t)tile * C * N;
+    for (int i = tid; i < C * N; i += NT) buf[(i / N) * CS + (i % N)] = row[i];
+    __syncthreads();
+    lds_fft<N, C, 1, CS, NT, false, 1>(buf, tw);
+    for (int i = tid; i < C * N; i += NT) row[i] = buf[(i / N) * CS + (i % N)];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// x pass (plain): columns along x, written back in place
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTX)
+k_xpass(float2* __restrict__ spec, const float2* __restrict__ twg) {
+  using Cf = FFTCfg<N>;
+  constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = (N / 2 + 1) * YT;
+  __shared__ float2 buf[N * PS];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int kz = tile / YT, yt = tile % YT;
+    float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
+    for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
+    __syncthreads();
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+    for (int i = tid; i < N * C; i += NT) base[(size_t)(i / C) * N + (i % C)] = buf[(i / C) * PS + (i % C)];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// bin lookup: slot of m = kx^2+ky^2+kz^2 under integer thresholds (see pk_api.h)
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int find_bin(int m, const int* __restrict__ thr, const int* __restrict__ jst, int nb) {
+  int j = (int)sqrtf((float)m);
+  if (j * j > m) --j;
+  else if ((j + 1) * (j + 1) <= m) ++j;
+  int b = jst[j];
+  while (b + 1 < nb && thr[b + 1] <= m) ++b;
+  if (b < 0 || m >= thr[nb]) return -1;
+  return b;
+}
+
+constexpr int HB_LDS = 256;   // bins held in LDS; above this the per-block slice lives in HBM
+constexpr int NACC = 5;       // S0, S2, S4, count, sum sqrt(m)
+
+// ------------------------------------------------------------------------------------
+// x pass fused with deconvolution, |delta_k|^2, Legendre weights and per-bin FP64 sums.
+// Each persistent block accumulates its own histogram and writes it to
+// partials[block][nb][NACC] once at the end.
+// ------------------------------------------------------------------------------------
+template <int N, bool LH>
+__global__ void __launch_bounds__(FFTCfg<N>::NTX)
+k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, const float* __restrict__ wtab,
+            const int* __restrict__ thr_g, const int* __restrict__ jst_g, int nb, int jmax, int flags,
+            double* __restrict__ partials) {
+  using Cf = FFTCfg<N>;
+  constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = (N / 2 + 1) * YT;
+  constexpr int H = N / 2;
+  __shared__ float2 buf[N * PS];
+  __shared__ float2 tw[N];
+  __shared__ float wt[N];
+  __shared__ int jst[isqrt_c(3 * H * H) + 2];
+  __shared__ int thr_s[LH ? HB_LDS + 1 : 1];
+  __shared__ double hs[LH ? 4 * HB_LDS : 1];
+  __shared__ unsigned int hc[LH ? HB_LDS : 1];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) {
+    tw[i] = twg[i];
+    wt[i] = wtab[i];
+  }
+  for (int i = tid; i <= jmax; i += NT) jst[i] = jst_g[i];
+  double* gpart = partials + (size_t)blockIdx.x * nb * NACC;
+  if constexpr (LH) {
+    for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
+    for (int i = tid; i < 4 * HB_LDS; i += NT) hs[i] = 0.0;
+    for (int i = tid; i < HB_LDS; i += NT) hc[i] = 0u;
+  } else {
+    for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;
+  }
+  __syncthreads();
+  const int* thr = LH ? thr_s : thr_g;
+  const bool want_k = (flags & PK_FLAG_KMEAN) != 0;
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int kz = tile / YT, yt = tile % YT;
+    const float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
+    for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
+    __syncthreads();
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+    const float wz = wt[kz];
+    const int kz2 = kz * kz;
+    // pair tasks: |kx| = q in [0, N/2], column c; entries q and N-q share m and mu
+    for (int i = tid; i < (H + 1) * C; i += NT) {
+      const int c = i % C, q = i / C;
+      const int y = yt * C + c;
+      const int ky = y <= H ? y : y - N;
+      const int m = q * q + ky * ky + kz2;
+      const int slot = find_bin(m, thr, jst, nb);
+      if (slot < 0) continue;
+      const float2 a = buf[q * PS + c];
+      float p2 = a.x * a.x + a.y * a.y;
+      int cnt = 1;
+      if (q != 0 && q != H) {
+        const float2 bb = buf[(N - q) * PS + c];
+        p2 += bb.x * bb.x + bb.y * bb.y;
+        cnt = 2;
+      }
+      p2 *= wt[q] * wt[y] * wz;
+      // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
+      //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
+      // so L2 is exactly 0 where 3 kz^2 = m.  The k = 0 mode's power is added by the
+      // finalize from the FP64 sum of the mesh (mu = 0 there, :37).
+      float l2 = 0.0f, l4 = 0.0f;
+      if (m > 0) {
+        const float rm = 1.0f / (float)m;
+        l2 = 0.5f * (float)(3 * kz2 - m) * rm;
+        const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
+        l4 = 0.125f * (float)num4 * rm * rm;
+      } else {
+        p2 = 0.0f;
+      }
+      if constexpr (LH) {
+        atomicAdd(&hs[slot], (double)p2);
+        atomicAdd(&hs[HB_LDS + slot], (double)(p2 * l2));
+        atomicAdd(&hs[2 * HB_LDS + slot], (double)(p2 * l4));
+        atomicAdd(&hc[slot], (unsigned)cnt);
+        if (want_k) atomicAdd(&hs[3 * HB_LDS + slot], (double)cnt * (double)sqrtf((float)m));
+      } else {
+        double* g = gpart + (size_t)slot * NACC;
+        atomicAdd(g + 0, (double)p2);
+        atomicAdd(g + 1, (double)(p2 * l2));
+        atomicAdd(g + 2, (double)(p2 * l4));
+        atomicAdd(g + 3, (double)cnt);
+        if (want_k) atomicAdd(g + 4, (double)cnt * (double)sqrtf((float)m));
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (LH) {
+    for (int i = tid; i < nb; i += NT) {
+      double* g = gpart + (size_t)i * NACC;
+      g[0] = hs[i];
+      g[1] = hs[HB_LDS + i];
+      g[2] = hs[2 * HB_LDS + i];
+      g[3] = (double)hc[i];
+      g[4] = want_k ? hs[3 * HB_LDS + i] : 0.0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// reduce the per-block partials in a fixed order and normalise (src/correlations.py:49-54)
+// ------------------------------------------------------------------------------------
+__global__ void k_bin_finalize(const double* __restrict__ partials, int G, int nb, const int* __restrict__ thr,
+                               const double* __restrict__ dc_part, int Gz, double norm, double shot,
+                               double* __restrict__ sums, int64_t* __restrict__ counts, float* __restrict__ pk,
+                               float* __restrict__ nmodes, double* __restrict__ kmean) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double acc[NACC] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int g = 0; g < G; ++g) {
+    const double* p = partials + ((size_t)g * nb + b) * NACC;
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) acc[a] += p[a];
+  }
+  // k = 0 lands in the last bin with thr[b] <= 0 (thresholds are non-decreasing), if kept
+  if (dc_part != nullptr && thr[b] <= 0 && (b + 1 == nb || thr[b + 1] > 0) && thr[nb] > 0) {
+    double dc = 0.0;
+    for (int g = 0; g < Gz; ++g) dc += dc_part[g];
+    const double p0 = dc * dc;   // mu = 0: L2 = -1/2, L4 = 3/8
+    acc[0] += p0;
+    acc[1] += -0.5 * p0;
+    acc[2] += 0.375 * p0;
+  }
+  const double cnt = acc[3];
+  if (sums) {
+    sums[b * 3 + 0] = acc[0];
+    sums[b * 3 + 1] = acc[1];
+    sums[b * 3 + 2] = acc[2];
+  }
+  if (counts) counts[b] = (int64_t)cnt;
+  if (nmodes) nmodes[b] = (float)cnt;
+  // 0/0 -> NaN for empty bins, as the reference (:50-52)
+  pk[b * 3 + 0] = (float)(acc[0] / cnt * norm - shot);
+  pk[b * 3 + 1] = (float)(acc[1] / cnt * 5.0 * norm);
+  pk[b * 3 + 2] = (float)(acc[2] / cnt * 9.0 * norm);
+  if (kmean) kmean[b] = acc[4] / cnt;
+}
+
+// ------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------
+template <typename K>
+static int persistent_grid(K kernel, int nt, int ntiles) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int cus = num_cus(dev);
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, nt, 0) != hipSuccess || occ < 1) occ = 1;
+  long g = (long)cus * occ;
+  if (g > ntiles) g = ntiles;
+  return (int)(g < 1 ? 1 : g);
+}
+
+template <int N> struct FFTLaunch {
+  using Cf = FFTCfg<N>;
+  static int zgrid() {
+    static int grid = 0;
+    if (!grid) grid = persistent_grid(k_zpass<N>, Cf::NTZ, N * (N / Cf::CZ));
+    return grid;
+  }
+  static int zpass(const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
+                   hipStream_t s) {
+    const int grid = zgrid();
+    StageTimer t(ST_ZPASS, s);
+    hipLaunchKernelGGL(k_zpass<N>, dim3(grid), dim3(Cf::NTZ), 0, s, mesh, spec, tw, stats, dc_part);
+    return check_hip(hipGetLastError(), "z pass launch");
+  }
+  static int ypass(float2* spec, const float2* tw, hipStream_t s) {
+    static int grid = 0;
+    if (!grid) grid = persistent_grid(k_ypass<N>, Cf::NTY, (N / 2 + 1) * N / Cf::CY);
+    StageTimer t(ST_YPASS, s);
+    hipLaunchKernelGGL(k_ypass<N>, dim3(grid), dim3(Cf::NTY), 0, s, spec, tw);
+    return check_hip(hipGetLastError(), "y pass launch");
+  }
+  static int xpass(float2* spec, const float2* tw, hipStream_t s) {
+    static int grid = 0;
+    if (!grid) grid = persistent_grid(k_xpass<N>, Cf::NTX, (N / 2 + 1) * (N / Cf::CX));
+    StageTimer t(ST_XPASS_PLAIN, s);
+    hipLaunchKernelGGL(k_xpass<N>, dim3(grid), dim3(Cf::NTX), 0, s, spec, tw);
+    return check_hip(hipGetLastError(), "x pass launch");
+  }
+  static int bin_grid(bool lh) {
+    static int g_l = 0, g_g = 0;
+    const int ntiles = (N / 2 + 1) * (N / Cf::CX);
+    if (lh) {
+      if (!g_l) g_l = persistent_grid(k_xpass_bin<N, true>, Cf::NTX, ntiles);
+      return g_l;
+    }
+    if (!g_g) g_g = persistent_grid(k_xpass_bin<N, false>, Cf::NTX, ntiles);
+    return g_g;
+  }
+  static int xpass_bin(const float2* spec, const float2* tw, const float* wtab, const int* thr, const int* jst,
+                       int nb, int jmax, int flags, double* partials, int grid, hipStream_t s) {
+    StageTimer t(ST_XPASS_BIN, s);
+    if (nb <= HB_LDS)
+      hipLaunchKernelGGL((k_xpass_bin<N, true>), dim3(grid), dim3(Cf::NTX), 0, s, spec, tw, wtab, thr, jst, nb, jmax,
+                         flags, partials);
+    else
+      hipLaunchKernelGGL((k_xpass_bin<N, false>), dim3(grid), dim3(Cf::NTX), 0, s, spec, tw, wtab, thr, jst, nb,
+                         jmax, flags, partials);
+    return check_hip(hipGetLastError(), "x pass + bin launch");
+  }
+};
+
+#define PK_FOR_EACH_N(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
+bool fft_supported(int n) {
+  switch (n) {
+#define PK_CASE(V) case V: return true;
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+    default: return false;
+  }
+}
+
+int zpass_grid(int n) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::zgrid();
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return 0;
+}
+
+int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
+              hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::zpass(mesh, spec, tw, stats, dc_part, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int fft_ypass(int n, float2* spec, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::ypass(spec, tw, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int fft_xpass(int n, float2* spec, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::xpass(spec, tw, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int bin_grid(int n, int nb) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::bin_grid(nb <= HB_LDS);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return 0;
+}
+
+int fft_xpass_bin(int n, const float2* spec, const float2* tw, const float* wtab, const int* thr, const int* jst,
+                  int nb, int jmax, int flags, double* partials, int grid, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::xpass_bin(spec, tw, wtab, thr, jst, nb, jmax, flags, partials, grid, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int bin_finalize(const double* partials, int G, int nb, const int* thr, const double* dc_part, int Gz, double norm,
+                 double shot, double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean, hipStream_t s) {
+  StageTimer t(ST_FINALIZE, s);
+  const int nt = 128;
+  hipLaunchKernelGGL(k_bin_finalize, dim3((nb + nt - 1) / nt), dim3(nt), 0, s, partials, G, nb, thr, dc_part, Gz,
+                     norm, shot, sums, counts, pk, nmodes, kmean);
+  return check_hip(hipGetLastError(), "bin finalize launch");
+}
+
+}  // namespace pk

@@ -1,0 +1,302 @@
+"""GPU parity of the HIP path (libpowspec_amd.so, through the C ABI) against the CPU oracle.
+
+Bars: mode counts and bin indices bit-exact (int64 equality); P_l within 1e-5 of the
+same-bin absolute scale A_l,b = (2l+1) sum_b |L_l(mu)| P2 norm / N_b (P_2, P_4 cross zero);
+meshes within float32 accumulation error; FFT within float32 rounding.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from jax_powspec_amd import binning, correlations, mas
+from oracle import ref_fp64 as R
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+PK_TOL = 1e-5
+
+
+def catalogue(rng, npart, box, clumps=True, edges=True):
+    pts = rng.random((npart, 3)) * box
+    if clumps:   # some clustering so P_2, P_4 are not pure noise
+        c = rng.random((20, 3)) * box
+        k = npart // 4
+        pts[:k] = (c[rng.integers(0, 20, k)] + rng.normal(0, box / 40, (k, 3))) % box
+        pts[: k // 2, 2] = (pts[: k // 2, 2] + rng.normal(0, box / 20, k // 2)) % box   # LOS stretch
+    if edges:
+        pts[:6] = [[0, 0, 0], [box * (1 - 1e-7), 1, 2], [3, box * (1 - 1e-7), 4], [5, 6, box - 1e-3],
+                   [box / 2, box / 2, box / 2], [0, box / 3, 0]]
+    return pts.astype(F32)
+
+
+def ref_catalogue(rng, npart, box, n):
+    """Adds particles outside [0, box) that exercise the reference's drop / wrap rules."""
+    p = catalogue(rng, npart, box)
+    H = box / n
+    extra = np.array([[box, 1, 1], [-0.4 * H, 2, 3], [-1.6 * H, 4, 5], [1.5 * box, 6, 7], [2 * box + 3, 8, 9],
+                      [1, -2.5 * H, box], [box * 1.0, box * 1.0, box * 1.0], [-box - 2 * H, 1, 1]], F32)
+    return np.concatenate([p, extra]).astype(F32)
+
+
+def mesh_close(g, o):
+    g = g.detach().cpu().numpy().astype(np.float64)
+    scale = np.max(np.abs(o))
+    err = np.abs(g - o)
+    assert np.all(err <= 4e-6 * scale + 2e-6 * np.abs(o)), (err.max(), scale)
+
+
+def pk_check(pk, nm, ex, o_pk, o_extra, box, n, shot=0.0, sum_abs=None):
+    """Gate on the GPU's FP64 sums; then check the float32 outputs are their rounding.
+
+    Bins holding only the k = 0 mode carry |sum delta|^2, a cancellation whose FP64 value
+    depends on summation order; they are compared as |sqrt S_gpu - sqrt S_o| <=
+    1e-12 sum|delta| instead of relatively.
+    """
+    cnt = ex["counts"].cpu().numpy()
+    np.testing.assert_array_equal(cnt, o_extra["counts"])
+    np.testing.assert_array_equal(nm.cpu().numpy(), o_extra["counts"].astype(np.float32))
+    S = ex["sums"].cpu().numpy()
+    norm = (box / n ** 2) ** 3
+    with np.errstate(invalid="ignore", divide="ignore"):
+        P = S / cnt[:, None] * norm * np.array([1.0, 5.0, 9.0])
+    o = np.array(o_pk, np.float64)
+    empty = cnt == 0
+    assert np.all(np.isnan(pk.cpu().numpy()[empty]))
+    check = ~empty
+    slot0 = int(o_extra["slot"][0])            # bin of the k = 0 mode (-1: dropped)
+    if slot0 >= 0 and cnt[slot0] == 1:
+        check[slot0] = False
+        So = o[slot0, 0] / norm
+        assert abs(np.sqrt(max(S[slot0, 0], 0)) - np.sqrt(max(So, 0))) <= 1e-12 * (sum_abs or 1.0)
+    scale = o_extra["scale"][check]
+    err = np.abs(P[check] - o[check]) / np.maximum(scale, 1e-300)
+    worst = float(np.nanmax(err)) if err.size else 0.0
+    assert worst <= PK_TOL, (worst, np.unravel_index(np.nanargmax(err), err.shape))
+    # float32 outputs = rounding of (S / N * norm * (2l+1), minus the shot noise on l = 0)
+    want32 = (P - np.array([shot, 0.0, 0.0])).astype(F32)
+    np.testing.assert_allclose(pk.cpu().numpy()[~empty], want32[~empty], rtol=2e-7, atol=0)
+    return worst
+
+
+# ----------------------------------------------------------------------------------------
+# mass assignment
+# ----------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [24, 32])
+@pytest.mark.parametrize("tile", [None, (4, 4, 8), (8, 6, 10), "direct"])
+@pytest.mark.parametrize("fn,wrap", [("vec", True), ("vec", False), ("scan", True), ("scan", False)])
+def test_reference_stencil_parity(gpu, n, tile, fn, wrap):
+    if tile == (8, 6, 10) and n == 32:
+        tile = (8, 6, 12)                     # leaves an 8-cell last tile on z
+    rng = np.random.default_rng(100 + n)
+    box = 200.0
+    p = ref_catalogue(rng, 5000, box, n)
+    w = (0.5 + rng.random(len(p))).astype(F32)
+    base = rng.standard_normal((n, n, n)).astype(F32)
+    t = torch.as_tensor(p, device=gpu)
+    tw = torch.as_tensor(w, device=gpu)
+    tb = torch.as_tensor(base, device=gpu)
+    mode = {("vec", True): 0, ("vec", False): 1, ("scan", True): 2, ("scan", False): 3}[(fn, wrap)]
+    got = mas._scatter(tb, t[:, 0], t[:, 1], t[:, 2], tw, 0.0, 0.0, 0.0, box, n, "CIC_REF", mode, tile=tile)
+    want = R._accumulate(base.astype(np.float64), n,
+                         R.cic_ref_deposits(p[:, 0], p[:, 1], p[:, 2], w, 0.0, 0.0, 0.0, box, n, mode))
+    mesh_close(got, want)
+    np.testing.assert_array_equal(tb.cpu().numpy(), base)      # functional: input untouched
+
+
+def test_dropin_signatures(gpu):
+    rng = np.random.default_rng(5)
+    n, box = 16, 64.0
+    p = catalogue(rng, 3000, box)
+    t = torch.as_tensor(p, device=gpu)
+    w = torch.ones(len(p), device=gpu)
+    z = torch.zeros((n, n, n), device=gpu)
+    for f, of in ((mas.cic_mas_vec, R.cic_mas_vec), (mas.cic_mas, R.cic_mas)):
+        for wrap in (True, False):
+            got = f(z, t[:, 0], t[:, 1], t[:, 2], w, len(p), 0.0, 0.0, 0.0, box, n, wrap)
+            want = of(np.zeros((n,) * 3), p[:, 0], p[:, 1], p[:, 2], np.ones(len(p), F32), len(p), 0., 0., 0., box,
+                      n, wrap)
+            mesh_close(got, want)
+    # xmin = particles[:, i].min() as tests/correlations.py:45 does
+    mins = [float(p[:, i].min()) for i in range(3)]
+    got = mas.cic_mas_vec(z, t[:, 0], t[:, 1], t[:, 2], w, len(p), *mins, box, n, True)
+    want = R.cic_mas_vec(np.zeros((n,) * 3), *p.T, np.ones(len(p), F32), len(p), *mins, box, n, True)
+    mesh_close(got, want)
+
+
+@pytest.mark.parametrize("stencil", ["CIC", "TSC"])
+@pytest.mark.parametrize("n,tile", [(16, None), (24, (4, 4, 8)), (32, (8, 8, 16)), (48, "direct"), (32, (2, 2, 2))])
+def test_cic_tsc_parity(gpu, stencil, n, tile):
+    rng = np.random.default_rng(7 + n)
+    box = 300.0
+    p = catalogue(rng, 20000, box)
+    w = (0.2 + rng.random(len(p))).astype(F32)
+    t = torch.as_tensor(p, device=gpu)
+    stats = torch.zeros(4, dtype=torch.float64, device=gpu)
+    got = mas.assign(t[:, 0], t[:, 1], t[:, 2], torch.as_tensor(w, device=gpu), box_size=box, n_bins=n, mas=stencil,
+                     tile=tile, stats=stats)
+    want = (R.cic_mesh if stencil == "CIC" else R.tsc_mesh)(*p.T, w, 0, 0, 0, box, n)
+    mesh_close(got, want)
+    m = stats.cpu().numpy()
+    assert m[0] == pytest.approx(float(w.astype(np.float64).sum()), rel=1e-6)
+    assert m[1] == pytest.approx(n ** 3 / m[0], rel=1e-12)
+
+
+def test_strided_aos_positions_and_dense_sorted_catalogue(gpu):
+    # (N, 3) array columns read in place (stride 3); a dense, cell-ordered catalogue
+    n, box = 16, 16.0
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    p = (np.repeat(g, 12, axis=0) + np.random.default_rng(0).random((n ** 3 * 12, 3))).astype(F32)
+    t = torch.as_tensor(p, device=gpu)
+    assert t[:, 1].stride(0) == 3
+    for st in ("CIC", "TSC", "CIC_REF"):
+        got = mas.assign(t[:, 0], t[:, 1], t[:, 2], box_size=box, n_bins=n, mas=st)
+        want = R.assign(st, *p.T, None, 0, 0, 0, box, n)
+        mesh_close(got, want)
+
+
+# ----------------------------------------------------------------------------------------
+# FFT
+# ----------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256])
+def test_rfftn_parity(gpu, n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n, n)).astype(F32) + F32(0.3)
+    got = correlations.rfftn(torch.as_tensor(a, device=gpu)).cpu().numpy().astype(np.complex128)
+    want = np.fft.rfftn(a.astype(np.float64))
+    rel2 = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert rel2 <= 1e-7 * max(1.0, math.log2(n ** 3)), rel2
+    assert np.max(np.abs(got - want)) <= 2e-6 * np.max(np.abs(want))
+
+
+def test_rfftn_plane_wave_exact_bins(gpu):
+    n = 64
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    a = np.cos(2 * np.pi * (3 * X - 5 * Y + 7 * Z) / n).astype(F32)
+    got = correlations.rfftn(torch.as_tensor(a, device=gpu)).cpu().numpy()
+    peak = n ** 3 / 2
+    assert abs(got[3, n - 5, 7] - peak) <= 1e-5 * peak
+    got[3, n - 5, 7] = 0
+    assert np.max(np.abs(got)) <= 2e-5 * peak
+
+
+# ----------------------------------------------------------------------------------------
+# P_l from a mesh
+# ----------------------------------------------------------------------------------------
+def _delta(rng, n, box):
+    p = catalogue(rng, 40 * n ** 2, box)
+    return R.contrast(R.cic_mesh(*p.T, np.ones(len(p), F32), 0, 0, 0, box, n)).astype(F32)
+
+
+EDGES = {
+    "kf_half": lambda n, kF: kF * (0.5 + np.arange(0, n // 2 + 1)),
+    "kf_integer_with_zero": lambda n, kF: kF * np.arange(0, n // 2 + 1),
+    "arbitrary": lambda n, kF: np.linspace(0.3 * kF, 0.9 * n / 2 * kF, 11),
+    "narrow_global_hist": lambda n, kF: np.arange(0.2 * kF, n / 2 * kF, 0.05 * kF),   # > 256 bins
+    "beyond_nyquist": lambda n, kF: np.arange(1e-4, 3 * n * kF, 1.7 * kF),
+}
+
+
+@pytest.mark.parametrize("n", [16, 24, 32, 64])
+@pytest.mark.parametrize("edges", sorted(EDGES))
+@pytest.mark.parametrize("window", ["CIC", "TSC"])
+def test_powspec_vec_parity(gpu, n, edges, window):
+    rng = np.random.default_rng(n * 7)
+    box = 750.0
+    d = _delta(rng, n, box)
+    kF = 2 * np.pi / box
+    k_edges = EDGES[edges](n, kF)
+    k3, pk, nm, ex = correlations.powspec_vec(torch.as_tensor(d, device=gpu), box, k_edges, mas=window, full=True)
+    p = 2 if window == "CIC" else 3
+    ok3, opk, onm, oex = R.powspec_vec(d, box, k_edges, p=p, full=True)
+    pk_check(pk, nm, ex, opk, oex, box, n, sum_abs=float(np.abs(d.astype(np.float64)).sum()))
+    np.testing.assert_allclose(k3.cpu().numpy(), ok3, rtol=1e-6)
+    km = ex["kmean"].cpu().numpy()
+    fin = oex["counts"] > 0
+    np.testing.assert_allclose(km[fin], oex["kmean"][fin], rtol=1e-6)
+
+
+@pytest.mark.parametrize("n", [16, 48, 64])
+def test_powspec_vec_fundamental_parity(gpu, n):
+    rng = np.random.default_rng(n)
+    box = 1000.0
+    d = _delta(rng, n, box)
+    k3, pk, nm, ex = correlations.powspec_vec_fundamental(torch.as_tensor(d, device=gpu), box, full=True)
+    ok3, opk, onm, oex = R.powspec_vec_fundamental(d, box, full=True)
+    pk_check(pk, nm, ex, opk, oex, box, n)
+    np.testing.assert_allclose(k3.cpu().numpy(), ok3, rtol=1e-6)
+
+
+# ----------------------------------------------------------------------------------------
+# whole path
+# ----------------------------------------------------------------------------------------
+@pytest.mark.parametrize("stencil", ["CIC_REF", "CIC", "TSC"])
+@pytest.mark.parametrize("n,kind", [(32, "kf_half"), (48, "fundamental"), (64, "kf_integer_with_zero")])
+def test_estimate_parity(gpu, stencil, n, kind):
+    rng = np.random.default_rng(n + len(stencil))
+    box = 1000.0
+    p = catalogue(rng, 60000, box)
+    kF = 2 * np.pi / box
+    k_edges = None if kind == "fundamental" else EDGES[kind](n, kF)
+    shot = box ** 3 / len(p)
+    t = torch.as_tensor(p, device=gpu)
+    k3, pk, nm, ex = correlations.estimate(t[:, 0], t[:, 1], t[:, 2], box_size=box, n_bins=n, k_edges=k_edges,
+                                           mas=stencil, shot_noise=shot, full=True)
+    window = 3 if stencil == "TSC" else 2
+    res = R.estimate(*p.T, None, box, n, k_edges, stencil=stencil, p=window, shot_noise=shot)
+    ok3, opk, onm, oex = res[:4]
+    # gate before the shot-noise subtraction; the float32 output includes it
+    pk_check(pk, nm, ex, opk + [shot, 0, 0], oex, box, n, shot=shot)
+
+
+def test_estimate_equals_two_step_path(gpu):
+    # estimate() == assign + (delta / mean - 1 in torch) + powspec_vec, at 128^3
+    rng = np.random.default_rng(42)
+    n, box = 128, 1000.0
+    p = torch.as_tensor(catalogue(rng, 1_000_000, box), device=gpu)
+    kF = 2 * np.pi / box
+    edges = kF * np.arange(1, n // 2 + 1)
+    k1, pk1, nm1, ex1 = correlations.estimate(p[:, 0], p[:, 1], p[:, 2], box_size=box, n_bins=n, k_edges=edges,
+                                              full=True)
+    m = mas.cic_mas_vec(torch.zeros((n, n, n), device=gpu), p[:, 0], p[:, 1], p[:, 2],
+                        torch.ones(p.shape[0], device=gpu), p.shape[0], 0.0, 0.0, 0.0, box, n, True)
+    d = (m.double() / m.double().mean() - 1.0).float()
+    k2, pk2, nm2, ex2 = correlations.powspec_vec(d, box, edges, full=True)
+    assert torch.equal(ex1["counts"], ex2["counts"])
+    scale = (ex2["sums"].abs() / ex2["counts"].unsqueeze(1)).cpu().numpy() * (box / n ** 2) ** 3 * [1, 5, 9]
+    err = np.abs(pk1.cpu().numpy() - pk2.cpu().numpy()) / scale[:, :1]
+    assert np.nanmax(err) < 2e-5
+
+
+def test_full_size_counts_and_determinism(gpu):
+    # 512^3, 2e7 particles: mode counts bit-exact vs a host lattice count, repeated runs agree
+    n, box, npart = 512, 1500.0, 20_000_000
+    g = torch.Generator(device=gpu)
+    g.manual_seed(2002)
+    p = torch.rand((npart, 3), device=gpu, generator=g) * box
+    kF = 2 * np.pi / box
+    edges = kF * (0.5 + np.arange(257))
+    outs = [correlations.estimate(p[:, 0], p[:, 1], p[:, 2], box_size=box, n_bins=n, k_edges=edges, full=True)
+            for _ in range(2)]
+    thr, _ = binning.thresholds_from_grid_edges(binning.grid_edges(edges, box), n)
+    ki = np.where(np.arange(n) > n // 2, np.arange(n) - n, np.arange(n))
+    cxy = (ki[:, None] ** 2 + ki[None, :] ** 2).reshape(-1)
+    hist_xy = np.bincount(cxy)
+    kz2 = np.arange(n // 2 + 1) ** 2
+    mult = np.zeros(binning.max_m(n) + 1, np.int64)
+    for v in kz2:
+        mult[v:v + hist_xy.size] += hist_xy
+    slots = binning.slots_for_m(np.arange(mult.size), thr)
+    want = np.bincount(slots[slots >= 0], weights=mult[slots >= 0], minlength=256)[:256].astype(np.int64)
+    for k3, pk, nm, ex in outs:
+        np.testing.assert_array_equal(ex["counts"].cpu().numpy(), want)
+    # repeated runs agree to 1e-6 of the same-bin scale (bounded by (2l+1) S_0 / N)
+    s0 = outs[0][3]["sums"][:, :1].abs()
+    d = (outs[0][3]["sums"] - outs[1][3]["sums"]).abs() / s0
+    assert float(d.max()) < 1e-6
+    # uniform particles: P_0 is shot noise times the reference stencil's aliasing; it must
+    # be positive, finite and of order L^3 / N_p everywhere
+    P0 = outs[0][1][:, 0].cpu().numpy()
+    assert np.all(np.isfinite(P0)) and np.all(P0 > 0.3 * box ** 3 / npart) and np.all(P0 < 30 * box ** 3 / npart)
