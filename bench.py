@@ -110,7 +110,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-timing", action="store_true", help="no per-stage events (for profiler runs)")
-    ap.add_argument("--tile", type=str, default=None, help="tx,ty,tz | direct")
+    ap.add_argument("--tile", type=str, default=None, help="1,ty,0 | direct")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 1 GiB device-to-device copy first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the path's "
@@ -123,6 +125,13 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.calib_copy:   # known-size copy: 1 GiB read + 1 GiB written per launch
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log("generating catalogue ...")
     pos = make_catalogue(N_MESH, BOX, NBAR, dev)
     npart = pos.shape[0]
@@ -133,6 +142,7 @@ def main():
     tile = None
     if args.tile:
         tile = "direct" if args.tile == "direct" else tuple(int(v) for v in args.tile.split(","))
+        log(f"tile {tile}")
     est = Estimator(box_size=BOX, n_bins=N_MESH, max_particles=npart, k_edges=edges, mas="CIC_REF",
                     shot_noise=shot, device=dev, tile=tile)
     x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]

@@ -66,7 +66,8 @@ extern "C" {
 #define PK_FLAG_KMEAN 2        /* also accumulate sum sqrt(m) per bin (mode-averaged k) */
 
 /* stats buffer (device double[4]) written by pk_scatter / pk_estimate:
- *   [0] deposited mass M, [1] n^3 / M (1 / mean density), [2] scratch, [3] scratch */
+ *   [0] deposited mass M, [1] n^3 / M (1 / mean density), [2] mass added by the global-
+ *   atomic path (scratch), [3] sum(w) / n^3 of the sorted path (scratch) */
 #define PK_STATS_LEN 4
 
 typedef struct pk_plan pk_plan;
@@ -84,8 +85,9 @@ int pk_supported_n(int n);   /* 1 if n is supported, else 0 */
  * Replaces cic_mas_vec (src/mas.py:88-153) and cic_mas (src/mas.py:5-87); CIC/TSC are
  * extensions.  mesh_out = mesh_in (or zeros if mesh_in is NULL) + deposits; mesh_out may
  * alias mesh_in.  Positions are element-strided (stride 3 reads an (N_p, 3) array in
- * place); w may be NULL (unit weights).  tile[3] = (tx, ty, tz) cell-tile of the sorted
- * LDS-privatised deposit; tile == NULL picks the default; tile[0] == 0 selects direct
+ * place); w may be NULL (unit weights).  tile[3] = (1, ty, n) row-bundle tile of the sorted
+ * LDS-privatised deposit (ty rows of full z extent; tile[2] may be 0 for n); tile == NULL
+ * picks the default (ty = largest divisor of n that is <= 32); tile[0] == 0 selects direct
  * global atomics.  stats (nullable) receives the deposited mass (PK_STATS_LEN doubles).
  * Index/fraction arithmetic follows the reference in float32: inv = 1 / ((float)box /
  * (float)n), pos = (x - xmin) * inv.

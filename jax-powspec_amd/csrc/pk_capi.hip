@@ -165,6 +165,29 @@ int check_bins(const int32_t* thr, const int32_t* jstart, int nb, const float* p
   return PK_OK;
 }
 
+// y pass, x pass + binning and finalize on a spectrum whose z pass is done
+int finish_powspec(pk_plan* p, float2* spec, const int32_t* thr, const int32_t* jstart, int nb, int window_p,
+                   int flags, double box, double shot, const double* dc_part, const double* scale_stats,
+                   double* partials, double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
+                   hipStream_t s) {
+  const int n = p->n;
+  int e = fft_ypass(n, spec, p->tw, s);
+  if (e) return e;
+  const int G = bin_grid(n, nb);
+  const int jmax = (int)floor(sqrt(3.0 * (n / 2) * (n / 2))) + 1;
+  e = fft_xpass_bin(n, spec, p->tw, p->wtab + (size_t)window_p * n, thr, jstart, nb, jmax, flags, partials, G, s);
+  if (e) return e;
+  const double h = box / ((double)n * (double)n);
+  const double norm = h * h * h;  // (box / n^2)^3, src/correlations.py:50-52
+  return bin_finalize(partials, G, nb, thr, dc_part, zpass_grid(n), scale_stats, norm, shot, sums, counts, pk,
+                      nmodes, (flags & PK_FLAG_KMEAN) ? kmean : nullptr, s);
+}
+
+double* dc_partials(void* ws, int n, int nb) {
+  return reinterpret_cast<double*>(static_cast<char*>(ws) +
+                                   align_up((size_t)bin_grid(n, nb) * (size_t)nb * 5 * sizeof(double)));
+}
+
 int run_powspec(pk_plan* p, const float* mesh, const double* stats, const int32_t* thr, const int32_t* jstart,
                 int nb, int window_p, int flags, double box, double shot, float2* spec, double* sums,
                 int64_t* counts, float* pk, float* nmodes, double* kmean, void* ws, size_t ws_bytes,
@@ -173,24 +196,14 @@ int run_powspec(pk_plan* p, const float* mesh, const double* stats, const int32_
   const size_t need = powspec_ws_bytes(n, nb);
   if (ws_bytes < need) return set_error(PK_E_WORKSPACE, "powspec workspace %zu < %zu bytes", ws_bytes, need);
   if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
-  const int G = bin_grid(n, nb);
-  const int Gz = zpass_grid(n);
   double* partials = static_cast<double*>(ws);
-  double* dc_part = reinterpret_cast<double*>(static_cast<char*>(ws) +
-                                              align_up((size_t)G * (size_t)nb * 5 * sizeof(double)));
+  double* dc_part = dc_partials(ws, n, nb);
   // with the contrast applied (ZERO_DC) the k = 0 mode is exactly 0 and no DC sum is needed
   const bool keep_dc = (flags & PK_FLAG_ZERO_DC) == 0;
   int e = fft_zpass(n, mesh, spec, p->tw, stats, keep_dc ? dc_part : nullptr, s);
   if (e) return e;
-  e = fft_ypass(n, spec, p->tw, s);
-  if (e) return e;
-  const int jmax = (int)floor(sqrt(3.0 * (n / 2) * (n / 2))) + 1;
-  e = fft_xpass_bin(n, spec, p->tw, p->wtab + (size_t)window_p * n, thr, jstart, nb, jmax, flags, partials, G, s);
-  if (e) return e;
-  const double h = box / ((double)n * (double)n);
-  const double norm = h * h * h;  // (box / n^2)^3, src/correlations.py:50-52
-  return bin_finalize(partials, G, nb, thr, keep_dc ? dc_part : nullptr, Gz, norm, shot, sums, counts, pk, nmodes,
-                      (flags & PK_FLAG_KMEAN) ? kmean : nullptr, s);
+  return finish_powspec(p, spec, thr, jstart, nb, window_p, flags, box, shot, keep_dc ? dc_part : nullptr, nullptr,
+                        partials, sums, counts, pk, nmodes, kmean, s);
 }
 }  // namespace
 
@@ -245,9 +258,10 @@ int pk_plan_destroy(pk_plan* p) {
 int pk_scatter_workspace_size(int n, int64_t np, int stencil, const int* tile, size_t* bytes) {
   if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
   if (n < 2 || n > 4096) return set_error(PK_E_INVALID, "mesh side n=%d out of range", n);
+  if (np < 0 || np > 0x7fffffffLL) return set_error(PK_E_INVALID, "particle count out of range");
   TileGeom g{};
   bool direct = false;
-  int e = make_tile_geom(n, stencil, tile, &g, &direct);
+  int e = make_tile_geom(n, np, stencil, tile, &g, &direct);
   if (e) return e;
   *bytes = scatter_workspace_bytes(g, direct, np);
   return PK_OK;
@@ -263,7 +277,7 @@ int pk_scatter(const float* x, int64_t sx, const float* y, int64_t sy, const flo
   if (!mesh_out) return set_error(PK_E_INVALID, "null mesh_out");
   TileGeom g{};
   bool direct = false;
-  e = make_tile_geom(n, stencil, tile, &g, &direct);
+  e = make_tile_geom(n, np, stencil, tile, &g, &direct);
   if (e) return e;
   return scatter_run(a, g, direct, mesh_in, mesh_out, stats, workspace, workspace_bytes, as_stream(stream));
 }
@@ -306,6 +320,7 @@ int pk_powspec_mesh(pk_plan* p, const float* mesh, const double* stats, const in
 
 namespace {
 struct EstimateWS {
+  bool fused;
   float* mesh;
   float2* spec;
   double* stats;
@@ -316,12 +331,12 @@ struct EstimateWS {
   size_t bytes;
 };
 
-int carve_estimate(pk_plan* p, int64_t np, int stencil, const int* tile, int nb, void* base, EstimateWS* w) {
+int carve_estimate(pk_plan* p, int64_t np, int stencil, const int* tile, int nb, void* base, EstimateWS* w,
+                   TileGeom* g, bool* direct) {
   const int n = p->n;
-  TileGeom g{};
-  bool direct = false;
-  int e = make_tile_geom(n, stencil, tile, &g, &direct);
+  int e = make_tile_geom(n, np, stencil, tile, g, direct);
   if (e) return e;
+  w->fused = !*direct && fused_supported(n, *g);
   char* b = static_cast<char*>(base);
   size_t off = 0;
   auto take = [&](size_t bytes) -> void* {
@@ -330,10 +345,10 @@ int carve_estimate(pk_plan* p, int64_t np, int stencil, const int* tile, int nb,
     return q;
   };
   const size_t n3 = (size_t)n * n * n;
-  w->mesh = static_cast<float*>(take(n3 * sizeof(float)));
+  w->mesh = w->fused ? nullptr : static_cast<float*>(take(n3 * sizeof(float)));
   w->spec = static_cast<float2*>(take((size_t)(n / 2 + 1) * n * n * sizeof(float2)));
   w->stats = static_cast<double*>(take(PK_STATS_LEN * sizeof(double)));
-  w->scatter_bytes = scatter_workspace_bytes(g, direct, np);
+  w->scatter_bytes = scatter_workspace_bytes(*g, *direct, np);
   w->scatter_ws = take(w->scatter_bytes);
   w->powspec_bytes = powspec_ws_bytes(n, nb);
   w->powspec_ws = take(w->powspec_bytes);
@@ -348,7 +363,9 @@ int pk_estimate_workspace_size(pk_plan* p, int64_t np, int stencil, const int* t
   if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
   if (nb < 1) return set_error(PK_E_INVALID, "need at least one bin (nb=%d)", nb);
   EstimateWS w{};
-  e = carve_estimate(p, np, stencil, tile, nb, nullptr, &w);
+  TileGeom g{};
+  bool direct = false;
+  e = carve_estimate(p, np, stencil, tile, nb, nullptr, &w, &g, &direct);
   if (e) return e;
   *bytes = w.bytes;
   return PK_OK;
@@ -368,18 +385,34 @@ int pk_estimate(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t 
   e = make_args(&a, x, sx, y, sy, z, sz, w, sw, np, xmin, ymin, zmin, box_size, p->n, stencil, wrap_mode);
   if (e) return e;
   EstimateWS ws{};
-  e = carve_estimate(p, np, stencil, tile, nb, workspace, &ws);
-  if (e) return e;
-  if (workspace_bytes < ws.bytes) return set_error(PK_E_WORKSPACE, "estimate workspace %zu < %zu bytes", workspace_bytes, ws.bytes);
   TileGeom g{};
   bool direct = false;
-  make_tile_geom(p->n, stencil, tile, &g, &direct);
+  e = carve_estimate(p, np, stencil, tile, nb, workspace, &ws, &g, &direct);
+  if (e) return e;
+  if (workspace_bytes < ws.bytes) return set_error(PK_E_WORKSPACE, "estimate workspace %zu < %zu bytes", workspace_bytes, ws.bytes);
+  if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
   hipStream_t s = as_stream(stream);
   double* st = stats ? stats : ws.stats;
-  e = scatter_run(a, g, direct, nullptr, ws.mesh, st, ws.scatter_ws, ws.scatter_bytes, s);
+  if (!ws.fused) {
+    e = scatter_run(a, g, direct, nullptr, ws.mesh, st, ws.scatter_ws, ws.scatter_bytes, s);
+    if (e) return e;
+    return run_powspec(p, ws.mesh, st, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size, shot_noise,
+                       ws.spec, sums, counts, pk, nmodes, kmean, ws.powspec_ws, ws.powspec_bytes, s);
+  }
+  // fused: sort -> deposit + z pass -> mass -> y pass -> x pass + bin -> finalize scaled by (n^3 / M)^2
+  const void* sorted = nullptr;
+  const int* key_start = nullptr;
+  double* tile_mass = nullptr;
+  const double* aux = nullptr;
+  e = scatter_run(a, g, false, nullptr, nullptr, st, ws.scatter_ws, ws.scatter_bytes, s, &sorted, &key_start,
+                  &tile_mass, &aux);
   if (e) return e;
-  return run_powspec(p, ws.mesh, st, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size, shot_noise,
-                     ws.spec, sums, counts, pk, nmodes, kmean, ws.powspec_ws, ws.powspec_bytes, s);
+  e = fused_deposit_zpass(p->n, a, g, key_start, sorted, p->tw, st, aux, ws.spec, tile_mass, s);
+  if (e) return e;
+  e = mass_finalize(tile_mass, g.ntiles, st, p->n, s);
+  if (e) return e;
+  return finish_powspec(p, ws.spec, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size, shot_noise,
+                        nullptr, st, static_cast<double*>(ws.powspec_ws), sums, counts, pk, nmodes, kmean, s);
 }
 
 int pk_timing_enable(int on) {

@@ -203,4 +203,25 @@ __host__ __device__ constexpr int isqrt_c(int v) {
 __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
+__host__ __device__ constexpr int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Fixed-order (deterministic) block sum for any block size nt <= red's length.
+// Every thread of the block must call it; ends with a barrier, result in all threads.
+__device__ __forceinline__ double block_sum_f64(double v, double* red, int nt) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = pow2_ceil(nt) / 2; s > 0; s >>= 1) {
+    if (tid < s && tid + s < nt) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
 }  // namespace pk

@@ -81,13 +81,8 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
     __syncthreads();
   }
   if (dc_part != nullptr) {  // fixed-order tree: the k = 0 mode in FP64 (sum of all values)
-    red[tid] = dc;
-    __syncthreads();
-    for (int off = NT / 2; off > 0; off >>= 1) {
-      if (tid < off) red[tid] += red[tid + off];
-      __syncthreads();
-    }
-    if (tid == 0) dc_part[blockIdx.x] = red[0];
+    const double t = block_sum_f64(dc, red, NT);
+    if (tid == 0) dc_part[blockIdx.x] = t;
   }
 }
 
@@ -170,24 +165,22 @@ k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, con
   constexpr int YT = N / C;
   constexpr int NTILES = (N / 2 + 1) * YT;
   constexpr int H = N / 2;
+  // LDS: the tile, the S_0,2,4 histogram, counts and thresholds (<= 80 KB at N = 512 so two
+  // blocks share a CU); twiddles, window and isqrt-start tables are read through the caches
   __shared__ float2 buf[N * PS];
-  __shared__ float2 tw[N];
-  __shared__ float wt[N];
-  __shared__ int jst[isqrt_c(3 * H * H) + 2];
   __shared__ int thr_s[LH ? HB_LDS + 1 : 1];
-  __shared__ double hs[LH ? 4 * HB_LDS : 1];
+  __shared__ double hs[LH ? 3 * HB_LDS : 1];
   __shared__ unsigned int hc[LH ? HB_LDS : 1];
+  const float2* tw = twg;
+  const float* wt = wtab;
+  const int* jst = jst_g;
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += NT) {
-    tw[i] = twg[i];
-    wt[i] = wtab[i];
-  }
-  for (int i = tid; i <= jmax; i += NT) jst[i] = jst_g[i];
   double* gpart = partials + (size_t)blockIdx.x * nb * NACC;
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
-    for (int i = tid; i < 4 * HB_LDS; i += NT) hs[i] = 0.0;
+    for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0.0;
     for (int i = tid; i < HB_LDS; i += NT) hc[i] = 0u;
+    for (int i = tid; i < nb; i += NT) gpart[(size_t)i * NACC + 4] = 0.0;   // sum sqrt(m) stays in HBM
   } else {
     for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;
   }
@@ -237,7 +230,7 @@ k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, con
         atomicAdd(&hs[HB_LDS + slot], (double)(p2 * l2));
         atomicAdd(&hs[2 * HB_LDS + slot], (double)(p2 * l4));
         atomicAdd(&hc[slot], (unsigned)cnt);
-        if (want_k) atomicAdd(&hs[3 * HB_LDS + slot], (double)cnt * (double)sqrtf((float)m));
+        if (want_k) atomicAdd(gpart + (size_t)slot * NACC + 4, (double)cnt * (double)sqrtf((float)m));
       } else {
         double* g = gpart + (size_t)slot * NACC;
         atomicAdd(g + 0, (double)p2);
@@ -256,7 +249,6 @@ k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, con
       g[1] = hs[HB_LDS + i];
       g[2] = hs[2 * HB_LDS + i];
       g[3] = (double)hc[i];
-      g[4] = want_k ? hs[3 * HB_LDS + i] : 0.0;
     }
   }
 }
@@ -264,27 +256,40 @@ k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, con
 // ------------------------------------------------------------------------------------
 // reduce the per-block partials in a fixed order and normalise (src/correlations.py:49-54)
 // ------------------------------------------------------------------------------------
-__global__ void k_bin_finalize(const double* __restrict__ partials, int G, int nb, const int* __restrict__ thr,
-                               const double* __restrict__ dc_part, int Gz, double norm, double shot,
-                               double* __restrict__ sums, int64_t* __restrict__ counts, float* __restrict__ pk,
-                               float* __restrict__ nmodes, double* __restrict__ kmean) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  double acc[NACC] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int g = 0; g < G; ++g) {
+constexpr int FIN_NT = 256;
+
+__global__ void __launch_bounds__(FIN_NT)
+k_bin_finalize(const double* __restrict__ partials, int G, int nb, const int* __restrict__ thr,
+               const double* __restrict__ dc_part, int Gz, const double* __restrict__ scale_stats, double norm,
+               double shot, double* __restrict__ sums, int64_t* __restrict__ counts, float* __restrict__ pk,
+               float* __restrict__ nmodes, double* __restrict__ kmean) {
+  __shared__ double red[FIN_NT];
+  const int b = blockIdx.x;   // one block per bin
+  double part[NACC] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int g = threadIdx.x; g < G; g += FIN_NT) {
     const double* p = partials + ((size_t)g * nb + b) * NACC;
 #pragma unroll
-    for (int a = 0; a < NACC; ++a) acc[a] += p[a];
+    for (int a = 0; a < NACC; ++a) part[a] += p[a];
   }
+  double acc[NACC];
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) acc[a] = block_sum_f64(part[a], red, FIN_NT);
+  double dcp = 0.0;
+  for (int g = threadIdx.x; dc_part != nullptr && g < Gz; g += FIN_NT) dcp += dc_part[g];
+  const double dc = dc_part != nullptr ? block_sum_f64(dcp, red, FIN_NT) : 0.0;
+  if (threadIdx.x != 0) return;
+  // spectrum of rho - c (fused assignment): delta_k = rho_k * n^3 / M for k != 0
+  const double sc = scale_stats ? scale_stats[1] * scale_stats[1] : 1.0;
   // k = 0 lands in the last bin with thr[b] <= 0 (thresholds are non-decreasing), if kept
   if (dc_part != nullptr && thr[b] <= 0 && (b + 1 == nb || thr[b + 1] > 0) && thr[nb] > 0) {
-    double dc = 0.0;
-    for (int g = 0; g < Gz; ++g) dc += dc_part[g];
     const double p0 = dc * dc;   // mu = 0: L2 = -1/2, L4 = 3/8
     acc[0] += p0;
     acc[1] += -0.5 * p0;
     acc[2] += 0.375 * p0;
   }
+  acc[0] *= sc;
+  acc[1] *= sc;
+  acc[2] *= sc;
   const double cnt = acc[3];
   if (sums) {
     sums[b * 3 + 0] = acc[0];
@@ -434,11 +439,11 @@ int fft_xpass_bin(int n, const float2* spec, const float2* tw, const float* wtab
   return set_error(PK_E_INVALID, "unsupported n=%d", n);
 }
 
-int bin_finalize(const double* partials, int G, int nb, const int* thr, const double* dc_part, int Gz, double norm,
-                 double shot, double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean, hipStream_t s) {
+int bin_finalize(const double* partials, int G, int nb, const int* thr, const double* dc_part, int Gz,
+                 const double* scale_stats, double norm, double shot, double* sums, int64_t* counts, float* pk,
+                 float* nmodes, double* kmean, hipStream_t s) {
   StageTimer t(ST_FINALIZE, s);
-  const int nt = 128;
-  hipLaunchKernelGGL(k_bin_finalize, dim3((nb + nt - 1) / nt), dim3(nt), 0, s, partials, G, nb, thr, dc_part, Gz,
+  hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(FIN_NT), 0, s, partials, G, nb, thr, dc_part, Gz, scale_stats,
                      norm, shot, sums, counts, pk, nmodes, kmean);
   return check_hip(hipGetLastError(), "bin finalize launch");
 }

@@ -1,0 +1,153 @@
+// Fused mass assignment + z pass of the R2C FFT (the mesh is never written to HBM).
+//
+// One block per row-bundle tile (x = xt, ty rows in y, all z): the sorted particle records
+// of the tile and of the neighbour buckets that spill into it are deposited into the ty
+// rows held in LDS, the estimated mean density c = sum(w) / n^3 is subtracted, and each
+// row is transformed along z (length-n/2 complex FFT + split, as the stand-alone z pass).
+// The spectrum rows go to [kz][x][y] like the z pass, so the y and x passes follow
+// unchanged.
+//
+// Contrast (tests/lognormal.py:79-80): delta = rho / rhobar - 1.  Subtracting any constant
+// changes only the k = 0 mode, and rhobar = M / n^3 is only known once every tile has
+// deposited, so the rows are transformed as rho - c and the per-bin sums are scaled by
+// (n^3 / M)^2 afterwards; k = 0 is taken as exactly 0.  c removes the large k = 0 value
+// from the float32 butterflies.
+#include "pk_particles.h"
+
+namespace pk {
+
+template <int N> struct FusedCfg {
+  static constexpr int NC = N / 2;
+  static constexpr int CS = NC + 1;                      // complex row stride in LDS
+  static constexpr int TY = N <= 32 ? N : (N % 32 == 0 ? 32 : (N % 24 == 0 ? 24 : 16));
+  static constexpr int NT = cmin(1024, cmax(64, (TY * NC / 8) / 64 * 64));
+  // deposit into 64-bit fixed-point cells when they fit (the FFT rows alias them afterwards)
+  static constexpr bool FIXED = (size_t)TY * N * 8 <= FIXED_LDS_MAX;
+  static constexpr size_t FFT_LDS = (size_t)TY * CS * sizeof(float2);
+  static constexpr size_t LDS = FIXED ? cmax((int)((size_t)TY * N * 8), (int)FFT_LDS) : FFT_LDS;
+  static constexpr int PER = (TY * N + NT - 1) / NT;     // cells per thread in the conversion
+};
+
+template <int N>
+__global__ void __launch_bounds__(FusedCfg<N>::NT)
+k_fused_deposit_zpass(ScatterArgs a, TileGeom g, const int* __restrict__ key_start, const void* __restrict__ sorted,
+                      const float2* __restrict__ twg, const double* __restrict__ stats, const double* __restrict__ aux,
+                      float2* __restrict__ spec, double* __restrict__ tile_mass) {
+  using Cf = FusedCfg<N>;
+  constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
+  extern __shared__ float2 buf[];   // FFT rows: C rows of CS complex = 2 * CS floats per row
+  __shared__ float2 tw[N];
+  __shared__ int s_key[MAX_TILE_BUCKETS], s_base[MAX_TILE_BUCKETS], s_pre[MAX_TILE_BUCKETS + 1];
+  __shared__ int s_nbk;
+  __shared__ double red[NT];
+  float* rows = reinterpret_cast<float*>(buf);
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(buf);   // [C][N] when FIXED
+  const int tid = threadIdx.x;
+  const int t = blockIdx.x;
+  const int xt = t / g.nyt, yt = t % g.nyt;
+  const int y0 = yt * C;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  if constexpr (Cf::FIXED) {
+    for (int i = tid; i < C * N; i += NT) acc[i] = 0ull;
+  } else {
+    for (int i = tid; i < C * 2 * CS; i += NT) rows[i] = 0.0f;
+  }
+  gather_buckets(g, N, a.stencil == PK_STENCIL_TSC, a.stencil == PK_STENCIL_CIC_REF, xt, yt, key_start, s_key,
+                 s_base, s_pre, &s_nbk);
+  const float scale = (float)aux[0];
+  deposit_buckets(a, g, sorted, s_base, s_pre, s_nbk, NT, xt, y0, [&](int ly, int iz, float wt) {
+    if constexpr (Cf::FIXED) atomicAdd(&acc[ly * N + iz], to_fixed(wt, scale));
+    else atomicAdd(&rows[ly * 2 * CS + iz], wt);
+  });
+  __syncthreads();
+  // tile mass (FP64) and rho - c into the FFT rows
+  const float c0 = (float)stats[3];
+  double msum = 0.0;
+  if constexpr (Cf::FIXED) {
+    const double inv = aux[1];
+    float v[Cf::PER];
+#pragma unroll
+    for (int j = 0; j < Cf::PER; ++j) {
+      const int i = tid + j * NT;
+      if (i < C * N) {
+        const double q = (double)(long long)acc[i] * inv;
+        msum += q;
+        v[j] = (float)q - c0;
+      }
+    }
+    __syncthreads();   // every cell read before the float rows overwrite them
+#pragma unroll
+    for (int j = 0; j < Cf::PER; ++j) {
+      const int i = tid + j * NT;
+      if (i < C * N) rows[(i / N) * 2 * CS + (i % N)] = v[j];
+    }
+  } else {
+    for (int i = tid; i < C * N; i += NT) {
+      const int r = i / N, z = i - r * N;
+      float* p = &rows[r * 2 * CS + z];
+      const float v = *p;
+      msum += (double)v;
+      *p = v - c0;
+    }
+  }
+  const double m = block_sum_f64(msum, red, NT);   // ends with a barrier
+  if (tid == 0) tile_mass[t] = m;
+  // z transform of the C rows (pairs of reals as NC complex, then the split)
+  lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+  for (int i = tid; i < (NC + 1) * C; i += NT) {
+    const int c = i % C, k = i / C;
+    const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
+    const float2 zr = buf[c * CS + (k == 0 ? 0 : NC - k)];
+    const float2 zc = make_float2(zr.x, -zr.y);
+    const float2 fe = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    const float2 dd = csub(zk, zc);
+    const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
+    spec[((size_t)k * N + xt) * N + (size_t)y0 + c] = cadd(fe, cmul(tw[k], fo));
+  }
+}
+
+template <int N> struct FusedLaunch {
+  using Cf = FusedCfg<N>;
+  static int run(const ScatterArgs& a, const TileGeom& g, const int* key_start, const void* sorted,
+                 const float2* tw, const double* stats, const double* aux, float2* spec, double* tile_mass,
+                 hipStream_t s) {
+    static bool attr = false;
+    if (!attr && Cf::LDS > 65536) {
+      int e = check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fused_deposit_zpass<N>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)Cf::LDS),
+                        "LDS attribute");
+      if (e) return e;
+    }
+    attr = true;
+    StageTimer t(ST_DEPOSIT, s);
+    hipLaunchKernelGGL(k_fused_deposit_zpass<N>, dim3(g.ntiles), dim3(Cf::NT), Cf::LDS, s, a, g, key_start, sorted,
+                       tw, stats, aux, spec, tile_mass);
+    return check_hip(hipGetLastError(), "fused deposit + z pass launch");
+  }
+};
+
+#define PK_FOR_EACH_FUSED_N(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
+bool fused_supported(int n, const TileGeom& g) {
+  switch (n) {
+#define PK_CASE(V) \
+  case V: return g.ty == FusedCfg<V>::TY && g.fixed == (int)FusedCfg<V>::FIXED && FusedCfg<V>::LDS + 16384 <= 160 * 1024;
+    PK_FOR_EACH_FUSED_N(PK_CASE)
+#undef PK_CASE
+    default: return false;
+  }
+}
+
+int fused_deposit_zpass(int n, const ScatterArgs& a, const TileGeom& g, const int* key_start, const void* sorted,
+                        const float2* tw, const double* stats, const double* aux, float2* spec, double* tile_mass,
+                        hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FusedLaunch<V>::run(a, g, key_start, sorted, tw, stats, aux, spec, tile_mass, s);
+    PK_FOR_EACH_FUSED_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "fused path: unsupported n=%d", n);
+}
+
+}  // namespace pk

@@ -1,191 +1,28 @@
 // Particle -> mesh mass assignment: reference stencil (src/mas.py:88-153 and :5-87),
 // textbook CIC and TSC.
 //
-// Owner-computes tiled deposit:
-//   1. count:  each particle gets a key = (home tile, boundary class); home = the base cell
-//              (reference stencil / CIC) or the nearest grid point (TSC).  The boundary
-//              class records on which faces of its tile the stencil spills into a
-//              neighbouring tile.  Particles the tiles cannot take (reference stencil with an
-//              out-of-range base index) get the last key ("irregular").
-//   2. scan:   exclusive prefix sum of the key counts.
-//   3. place:  particles are written, key-sorted, as float4(xpos, ypos, zpos, w).
-//   4. deposit: one block per tile loads its cells into LDS (zeros or mesh_in), walks its
-//              own buckets plus the neighbour buckets whose class spills into it, adds
-//              every stencil corner that falls inside the tile with LDS float atomics and
-//              writes the tile once with coalesced stores: each mesh cell is written by
-//              exactly one block, with no global atomics and no separate zero fill.
+// Sorted owner-computes deposit on row-bundle tiles (1 x ty x n cells, full z rows):
+//   1. count:  each block takes a chunk of <= 65535 particles and histograms their keys
+//              (tile, y-class) in LDS (16-bit counters packed in pairs); the block's row of
+//              counts goes to a [blocks][keys] matrix.  No global atomics.
+//   2. scan:   a column scan of the matrix gives every (block, key) its offset inside the
+//              key; an exclusive scan of the key totals gives the key starts.
+//   3. place:  the blocks re-read their chunk and write float4(xpos, ypos, zpos, w) records
+//              at key start + block offset + LDS rank.
+//   4. deposit: one block per tile stages its ty full rows in LDS (zeros or mesh_in), walks
+//              its own buckets plus the neighbour buckets that spill into it (the whole
+//              x - 1 / x + 1 tiles, the boundary-row classes of the y neighbours), adds every
+//              stencil corner that falls inside with LDS float atomics and writes each row
+//              once with coalesced stores: no global atomics and no zero fill.
 //   5. atomic: the (normally empty) irregular bucket is added with global atomics.
-// A direct mode (tile[0] == 0) skips 1-4 and adds every particle with global atomics.
-//
-// The position -> (cell, fraction) arithmetic is float32 exactly as the reference
-// evaluates it on float32 inputs: pos = (x - xmin) * inv (src/mas.py:103-105),
-// ix = int32(pos) (truncation, :107-109), dd = pos - ix (:111-113), md = 1 - dd (:138-140).
-#include "pk_common.h"
-#include "pk_internal.h"
+// A direct mode (tile[0] == 0) adds every particle with global atomics instead.
+#include "pk_particles.h"
 
 namespace pk {
-
-constexpr float POS_LIMIT = 1073741824.0f;  // |pos| >= 2^30 or non-finite: particle dropped
-
-__device__ __forceinline__ int floor_mod(int a, int n) {
-  int r = a % n;
-  return r < 0 ? r + n : r;
-}
-
-__device__ __forceinline__ void load_particle(const ScatterArgs& a, int64_t i, float& px, float& py, float& pz,
-                                              float& w) {
-  px = (a.x[i * a.sx] - a.xmin) * a.inv;
-  py = (a.y[i * a.sy] - a.ymin) * a.inv;
-  pz = (a.z[i * a.sz] - a.zmin) * a.inv;
-  w = a.w ? a.w[i * a.sw] : 1.0f;
-}
-
-__device__ __forceinline__ bool pos_ok(float px, float py, float pz) {
-  return fabsf(px) < POS_LIMIT && fabsf(py) < POS_LIMIT && fabsf(pz) < POS_LIMIT;  // false for NaN
-}
-
-// one axis of the reference stencil: base index b, +1 index p (wrap rule of `mode`),
-// validity after JAX index normalisation, and fractions md, dd
-struct RefAxis {
-  int b, p;
-  bool vb, vp;
-  float md, dd;
-};
-
-__device__ __forceinline__ RefAxis ref_axis(float pos, int n, int mode) {
-  RefAxis r;
-  const int i = (int)pos;  // truncation toward zero
-  float d = pos - (float)i;
-  r.md = 1.0f - d;
-  int p = i + 1;
-  if (mode == PK_WRAP_VEC_TRUE) {
-    p = floor_mod(p + n, n);               // src/mas.py:121
-  } else if (mode == PK_WRAP_VEC_FALSE) {
-    p = (p >= n) ? 0 : p;                  // src/mas.py:125 (ddx test at :126 never fires)
-  } else if (mode == PK_WRAP_SCAN_TRUE) {
-    p = (p >= n) ? p - n : p;              // src/mas.py:24
-  } else {                                 // PK_WRAP_SCAN_FALSE, src/mas.py:34
-    if (p >= n) {
-      p = 0;
-      d = 0.0f;
-    }
-  }
-  r.dd = d;
-  r.b = i < 0 ? i + n : i;
-  r.vb = (r.b >= 0) && (r.b < n);
-  r.p = p < 0 ? p + n : p;
-  r.vp = (r.p >= 0) && (r.p < n);
-  return r;
-}
-
-// home cell per axis (regular particles only)
-__device__ __forceinline__ int home_cell(float pos, int n, int stencil) {
-  if (stencil == PK_STENCIL_TSC) return floor_mod((int)floorf(pos + 0.5f), n);
-  if (stencil == PK_STENCIL_CIC) return floor_mod((int)floorf(pos), n);
-  const int i = (int)pos;
-  return i < 0 ? i + n : i;
-}
-
-// key of a particle, -1 if dropped
-__device__ __forceinline__ int particle_key(const ScatterArgs& a, const TileGeom& g, float px, float py, float pz) {
-  if (!pos_ok(px, py, pz)) return -1;
-  const int n = a.n;
-  if (a.stencil == PK_STENCIL_CIC_REF) {
-    const int ix = (int)px, iy = (int)py, iz = (int)pz;
-    const int bx = ix < 0 ? ix + n : ix, by = iy < 0 ? iy + n : iy, bz = iz < 0 ? iz + n : iz;
-    if (bx < 0 || bx >= n || by < 0 || by >= n || bz < 0 || bz >= n) return g.nkeys - 1;
-  }
-  const float pos[3] = {px, py, pz};
-  int tile = 0, cls = 0, mul = 1;
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    const int c = home_cell(pos[ax], n, a.stencil);
-    const int t = c / g.td[ax];
-    const int local = c - t * g.td[ax];
-    const int ext = min(g.td[ax], n - t * g.td[ax]);
-    tile = tile * g.nt[ax] + t;
-    int v;
-    if (a.stencil == PK_STENCIL_TSC) {
-      v = (local == 0) ? 1 : ((local == ext - 1) ? 2 : 0);
-      cls += v * mul;
-      mul *= 3;
-    } else {
-      v = (local == ext - 1) ? 1 : 0;
-      cls += v * mul;
-      mul *= 2;
-    }
-  }
-  return tile * g.ncls + cls;
-}
-
-// Visit every stencil corner of one particle: f(ix, iy, iz, weight); indexes are in
-// [0, n) for regular particles.  For the reference stencil `all_checks` applies the
-// per-corner JAX drop rule (used by the atomic path for irregular particles).
-template <typename F>
-__device__ __forceinline__ void for_each_corner(const ScatterArgs& a, float px, float py, float pz, float w, F&& f) {
-  const int n = a.n;
-  if (a.stencil == PK_STENCIL_CIC_REF) {
-    const RefAxis X = ref_axis(px, n, a.wrap_mode), Y = ref_axis(py, n, a.wrap_mode), Z = ref_axis(pz, n, a.wrap_mode);
-    // src/mas.py:142-151, products evaluated left to right as in the reference
-    if (X.vb && Y.vb && Z.vb) f(X.b, Y.b, Z.b, X.md * Y.md * Z.md * w);
-    if (X.vp && Y.vb && Z.vb) f(X.p, Y.b, Z.b, X.dd * Y.md * Z.md * w);
-    if (X.vb && Y.vp && Z.vb) f(X.b, Y.p, Z.b, X.md * Y.dd * Z.md * w);
-    if (X.vb && Y.vb && Z.vp) f(X.b, Y.b, Z.p, X.md * Y.md * Z.dd * w);
-    if (X.vp && Y.vp && Z.vb) f(X.p, Y.p, Z.b, X.dd * Y.dd * Z.md * w);
-    if (X.vp && Y.vb && Z.vp) f(X.p, Y.b, Z.p, X.dd * Y.md * Z.dd * w);
-    if (X.vb && Y.vp && Z.vp) f(X.b, Y.p, Z.p, X.md * Y.md * Z.dd * w);  // src/mas.py:149 weight
-    if (X.vp && Y.vp && Z.vp) f(X.p, Y.p, Z.p, X.dd * Y.dd * Z.dd * w);
-  } else if (a.stencil == PK_STENCIL_CIC) {
-    const float pos[3] = {px, py, pz};
-    int c0[3], c1[3];
-    float w0[3], w1[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      const float f0 = floorf(pos[ax]);
-      const float d = pos[ax] - f0;
-      c0[ax] = floor_mod((int)f0, n);
-      c1[ax] = c0[ax] + 1 == n ? 0 : c0[ax] + 1;
-      w0[ax] = 1.0f - d;
-      w1[ax] = d;
-    }
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int sx = s >> 2, sy = (s >> 1) & 1, sz = s & 1;
-      f(sx ? c1[0] : c0[0], sy ? c1[1] : c0[1], sz ? c1[2] : c0[2],
-        (sx ? w1[0] : w0[0]) * (sy ? w1[1] : w0[1]) * (sz ? w1[2] : w0[2]) * w);
-    }
-  } else {  // TSC
-    const float pos[3] = {px, py, pz};
-    int c[3][3];
-    float wt[3][3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      const float f0 = floorf(pos[ax] + 0.5f);
-      const float d = pos[ax] - f0;
-      const int i0 = floor_mod((int)f0, n);
-      c[ax][0] = i0 == 0 ? n - 1 : i0 - 1;
-      c[ax][1] = i0;
-      c[ax][2] = i0 + 1 == n ? 0 : i0 + 1;
-      const float hm = 0.5f - d, hp = 0.5f + d;
-      wt[ax][0] = 0.5f * hm * hm;
-      wt[ax][1] = 0.75f - d * d;
-      wt[ax][2] = 0.5f * hp * hp;
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) f(c[0][i], c[1][j], c[2][k], wt[0][i] * wt[1][j] * wt[2][k] * w);
-  }
-}
 
 // ------------------------------------------------------------------------------------
 // small utility kernels (fills instead of hipMemset so a captured graph has no memset node)
 // ------------------------------------------------------------------------------------
-__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
-}
 __global__ void k_fill_f64(double* __restrict__ p, int n, double v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -195,21 +32,61 @@ __global__ void k_copy_or_zero(const float* src, float* dst, int64_t n) {  // sr
     dst[i] = src ? src[i] : 0.0f;
 }
 
+constexpr int CNT_NT = 1024;
+
 // ------------------------------------------------------------------------------------
-// 1. count
+// 1. count: LDS histogram of one chunk -> matrix row; FP64 weight sum per block
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_sort_count(ScatterArgs a, TileGeom g, int* __restrict__ counts) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.np; i += (int64_t)gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(CNT_NT) k_bucket_count(ScatterArgs a, TileGeom g, unsigned* __restrict__ mat,
+                                                         double* __restrict__ wsum_part,
+                                                         double* __restrict__ wmax_part) {
+  extern __shared__ unsigned hist2[];   // key k -> 16-bit half (k & 1) of word k >> 1
+  __shared__ double red[CNT_NT];
+  __shared__ unsigned wmax_bits;
+  const int tid = threadIdx.x;
+  const int words = (g.nkeys + 1) >> 1;
+  for (int i = tid; i < words; i += CNT_NT) hist2[i] = 0u;
+  if (tid == 0) wmax_bits = 0u;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * g.chunk;
+  const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
+  double ws = 0.0;
+  float wm = 0.0f;
+  for (int64_t i = lo + tid; i < hi; i += CNT_NT) {
     float px, py, pz, w;
     load_particle(a, i, px, py, pz, w);
-    const int key = particle_key(a, g, px, py, pz);
-    if (key >= 0) atomicAdd(&counts[key], 1);
+    const int key = row_tile_key(a, g, px, py, pz);
+    if (key >= 0) {
+      atomicAdd(&hist2[key >> 1], 1u << (16 * (key & 1)));
+      ws += (double)w;
+      wm = fmaxf(wm, fabsf(w));
+    }
   }
+  atomicMax(&wmax_bits, __float_as_uint(wm));   // non-negative floats order like their bits
+  __syncthreads();
+  if (tid == 0) wmax_part[blockIdx.x] = (double)__uint_as_float(wmax_bits);
+  unsigned* row = mat + (size_t)blockIdx.x * g.nkeys;
+  for (int k = tid; k < g.nkeys; k += CNT_NT) row[k] = (hist2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+  const double t = block_sum_f64(ws, red, CNT_NT);
+  if (tid == 0) wsum_part[blockIdx.x] = t;
 }
 
 // ------------------------------------------------------------------------------------
-// 2. exclusive scan (3 kernels; up to 4096 * 4096 keys)
+// 2. column scan (offset of each block inside each key) + exclusive scan of key totals
 // ------------------------------------------------------------------------------------
+__global__ void k_col_scan(unsigned* __restrict__ mat, int nblk, int nkeys, int* __restrict__ totals) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nkeys) return;
+  unsigned run = 0;
+  for (int b = 0; b < nblk; ++b) {
+    unsigned* p = mat + (size_t)b * nkeys + k;
+    const unsigned c = *p;
+    *p = run;
+    run += c;
+  }
+  totals[k] = (int)run;
+}
+
 constexpr int SCAN_NT = 1024;
 constexpr int SCAN_PER = 4;
 constexpr int SCAN_BLOCK = SCAN_NT * SCAN_PER;
@@ -251,8 +128,15 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_local(const int* __restrict__ 
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(SCAN_NT) k_scan_top(int* __restrict__ bsum, int nblk, int* __restrict__ total_out) {
+// scans the block sums; also sums the weight partials into stats[3] = sum(w) / n^3
+__global__ void __launch_bounds__(SCAN_NT) k_scan_top(int* __restrict__ bsum, int nblk, int* __restrict__ total_out,
+                                                      const double* __restrict__ wsum_part,
+                                                      const double* __restrict__ wmax_part, int nw,
+                                                      double* __restrict__ stats, double* __restrict__ aux,
+                                                      double ncells) {
   __shared__ int sh[SCAN_NT];
+  __shared__ double shd[SCAN_NT];
+  __shared__ unsigned long long wmax_bits;
   const int base = threadIdx.x * SCAN_PER;
   int v[SCAN_PER];
   int s = 0;
@@ -269,182 +153,122 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_top(int* __restrict__ bsum, in
     run += v[k];
   }
   if (threadIdx.x == 0) *total_out = total;
+  double ws = 0.0, wm = 0.0;
+  if (threadIdx.x == 0) wmax_bits = 0ull;
+  for (int i = threadIdx.x; i < nw; i += SCAN_NT) {
+    ws += wsum_part[i];
+    wm = fmax(wm, wmax_part[i]);
+  }
+  const double t = block_sum_f64(ws, shd, SCAN_NT);   // ends with a barrier
+  atomicMax(&wmax_bits, (unsigned long long)__double_as_longlong(wm));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    stats[3] = t / ncells;
+    // fixed-point scale S = 2^32 / 2^ceil(log2 max|w|): every deposit |rint(w S)| <= 2^32
+    int e = 0;
+    const double wmax = __longlong_as_double((long long)wmax_bits);
+    if (wmax > 0.0 && wmax < 1e300) (void)frexp(wmax, &e);
+    aux[0] = ldexp(1.0, 32 - e);
+    aux[1] = ldexp(1.0, e - 32);
+  }
 }
 
-__global__ void __launch_bounds__(SCAN_NT) k_scan_add(int* __restrict__ out, int* __restrict__ cursor,
-                                                      const int* __restrict__ bsum, int n) {
+__global__ void __launch_bounds__(SCAN_NT) k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) {
   const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER;
   const int add = bsum[blockIdx.x];
 #pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k) {
-    if (base + k < n) {
-      const int o = out[base + k] + add;
-      out[base + k] = o;
-      cursor[base + k] = o;
-    }
-  }
+  for (int k = 0; k < SCAN_PER; ++k)
+    if (base + k < n) out[base + k] += add;
 }
 
 // ------------------------------------------------------------------------------------
-// 3. place (key-sorted float4 records)
+// 3. place: key start + block offset + LDS rank
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_sort_place(ScatterArgs a, TileGeom g, int* __restrict__ cursor,
-                                                    float4* __restrict__ sorted) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.np; i += (int64_t)gridDim.x * blockDim.x) {
-    float px, py, pz, w;
-    load_particle(a, i, px, py, pz, w);
-    const int key = particle_key(a, g, px, py, pz);
-    if (key >= 0) {
-      const int at = atomicAdd(&cursor[key], 1);
-      sorted[at] = make_float4(px, py, pz, w);
-    }
-  }
-}
+constexpr int PLACE_UNROLL = 4;
 
-// ------------------------------------------------------------------------------------
-// 4. tile deposit (one block per tile)
-// ------------------------------------------------------------------------------------
-constexpr int DEP_NT = 256;
-constexpr int MAX_BUCKETS = 125;
-
-__device__ double block_sum_f64(double v, double* sh) {
-  // fixed-order tree: deterministic
+__global__ void __launch_bounds__(CNT_NT) k_bucket_place(ScatterArgs a, TileGeom g, const unsigned* __restrict__ mat,
+                                                         const int* __restrict__ key_start,
+                                                         void* __restrict__ sorted) {
+  extern __shared__ unsigned dest[];   // next write position of every key for this block
   const int tid = threadIdx.x;
-  sh[tid] = v;
+  const unsigned* row = mat + (size_t)blockIdx.x * g.nkeys;
+  for (int k = tid; k < g.nkeys; k += CNT_NT) dest[k] = (unsigned)key_start[k] + row[k];
   __syncthreads();
-  for (int off = DEP_NT / 2; off > 0; off >>= 1) {
-    if (tid < off) sh[tid] += sh[tid + off];
-    __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * g.chunk;
+  const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
+  const bool has_w = a.w != nullptr;
+  // PLACE_UNROLL independent particles per thread and iteration keep several loads in flight
+  for (int64_t i0 = lo + tid; i0 < hi; i0 += (int64_t)CNT_NT * PLACE_UNROLL) {
+    float px[PLACE_UNROLL], py[PLACE_UNROLL], pz[PLACE_UNROLL], w[PLACE_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PLACE_UNROLL; ++u) {
+      const int64_t i = i0 + (int64_t)u * CNT_NT;
+      if (i < hi) load_particle(a, i, px[u], py[u], pz[u], w[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < PLACE_UNROLL; ++u) {
+      const int64_t i = i0 + (int64_t)u * CNT_NT;
+      if (i >= hi) continue;
+      const int key = row_tile_key(a, g, px[u], py[u], pz[u]);
+      if (key >= 0) store_record(sorted, atomicAdd(&dest[key], 1u), has_w, px[u], py[u], pz[u], w[u]);
+    }
   }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
 }
 
-__global__ void __launch_bounds__(DEP_NT) k_deposit_tiles(ScatterArgs a, TileGeom g, const int* __restrict__ offsets,
-                                                          const float4* __restrict__ sorted,
-                                                          const float* mesh_in, float* mesh_out,
-                                                          double* __restrict__ tile_mass) {
-  extern __shared__ float tilebuf[];
-  __shared__ int s_pc[3][5], s_pk[3][5], s_np[3];
-  __shared__ int s_base[MAX_BUCKETS], s_pre[MAX_BUCKETS + 1];
+// ------------------------------------------------------------------------------------
+// 4. deposit (mesh mode): one block per row-bundle tile
+// ------------------------------------------------------------------------------------
+constexpr int DEP_NT = 512;
+
+template <bool FIXED>
+__global__ void __launch_bounds__(DEP_NT) k_rows_deposit(ScatterArgs a, TileGeom g, const int* __restrict__ key_start,
+                                                         const void* __restrict__ sorted, const double* __restrict__ aux,
+                                                         const float* mesh_in, float* mesh_out,
+                                                         double* __restrict__ tile_mass) {
+  extern __shared__ unsigned long long acc_q[];   // [ty][n]: 64-bit fixed point, or float when !FIXED
+  float* acc_f = reinterpret_cast<float*>(acc_q);
+  __shared__ int s_key[MAX_TILE_BUCKETS], s_base[MAX_TILE_BUCKETS], s_pre[MAX_TILE_BUCKETS + 1];
   __shared__ int s_nbk;
-  __shared__ double s_red[DEP_NT];
+  __shared__ double red[DEP_NT];
   const int tid = threadIdx.x;
   const int n = a.n;
-  const int tile = blockIdx.x;
-  int tc[3];
-  tc[2] = tile % g.nt[2];
-  tc[1] = (tile / g.nt[2]) % g.nt[1];
-  tc[0] = tile / (g.nt[2] * g.nt[1]);
-  int org[3], ext[3];
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    org[ax] = tc[ax] * g.td[ax];
-    ext[ax] = min(g.td[ax], n - org[ax]);
-  }
-  const int tdy = g.td[1], tdz = g.td[2];
-  const int cells = g.td[0] * tdy * tdz;
-  // init the tile (cells outside ext are never touched); mesh_in may alias mesh_out, so it
-  // is read only here, before this block writes any of its cells
-  double minit = 0.0;
+  const int t = blockIdx.x;
+  const int xt = t / g.nyt, yt = t % g.nyt;
+  const int y0 = yt * g.ty;
+  const int cells = g.ty * n;
+  const size_t gbase = ((size_t)xt * n + y0) * n;   // rows y0 .. y0+ty-1 are contiguous
   for (int i = tid; i < cells; i += DEP_NT) {
-    const int lz = i % tdz, ly = (i / tdz) % tdy, lx = i / (tdz * tdy);
-    float v = 0.0f;
-    if (mesh_in && lx < ext[0] && ly < ext[1] && lz < ext[2])
-      v = mesh_in[((size_t)(org[0] + lx) * n + (org[1] + ly)) * n + (org[2] + lz)];
-    tilebuf[i] = v;
-    minit += (double)v;
+    if (FIXED) acc_q[i] = 0ull;
+    else acc_f[i] = 0.0f;
   }
-  // per-axis (tile coordinate, class) pairs whose particles reach this tile
-  if (tid == 0) {
-    const bool tsc = a.stencil == PK_STENCIL_TSC;
-    int total = 1;
-    for (int ax = 0; ax < 3; ++ax) {
-      const int t = tc[ax], nt = g.nt[ax];
-      int k = 0;
-      if (tsc) {
-        for (int v = 0; v < 3; ++v) { s_pc[ax][k] = t; s_pk[ax][k] = v; ++k; }
-        if (nt >= 3) {
-          s_pc[ax][k] = (t + nt - 1) % nt; s_pk[ax][k] = 2; ++k;   // lower tile, last layer
-          s_pc[ax][k] = (t + 1) % nt;      s_pk[ax][k] = 1; ++k;   // upper tile, first layer
-        } else if (nt == 2) {
-          s_pc[ax][k] = 1 - t; s_pk[ax][k] = 1; ++k;
-          s_pc[ax][k] = 1 - t; s_pk[ax][k] = 2; ++k;
-        }
-      } else {
-        for (int v = 0; v < 2; ++v) { s_pc[ax][k] = t; s_pk[ax][k] = v; ++k; }
-        if (nt >= 2) { s_pc[ax][k] = (t + nt - 1) % nt; s_pk[ax][k] = 1; ++k; }
-      }
-      s_np[ax] = k;
-      total *= k;
-    }
-    s_nbk = total;
-  }
+  gather_buckets(g, n, a.stencil == PK_STENCIL_TSC, false, xt, yt, key_start, s_key, s_base, s_pre, &s_nbk);
+  const float scale = (float)aux[0];
+  deposit_buckets(a, g, sorted, s_base, s_pre, s_nbk, DEP_NT, xt, y0, [&](int ly, int iz, float wt) {
+    if (FIXED) atomicAdd(&acc_q[ly * n + iz], to_fixed(wt, scale));
+    else atomicAdd(&acc_f[ly * n + iz], wt);
+  });
   __syncthreads();
-  const int nbk = s_nbk;
-  int mysize = 0;
-  if (tid < nbk) {
-    const int i2 = tid % s_np[2], i1 = (tid / s_np[2]) % s_np[1], i0 = tid / (s_np[2] * s_np[1]);
-    const int nt1 = g.nt[1], nt2 = g.nt[2];
-    const int tl = (s_pc[0][i0] * nt1 + s_pc[1][i1]) * nt2 + s_pc[2][i2];
-    const bool tsc = a.stencil == PK_STENCIL_TSC;
-    const int cls = tsc ? s_pk[0][i0] + 3 * s_pk[1][i1] + 9 * s_pk[2][i2] : s_pk[0][i0] + 2 * s_pk[1][i1] + 4 * s_pk[2][i2];
-    const int key = tl * g.ncls + cls;
-    const int lo = offsets[key];
-    mysize = offsets[key + 1] - lo;
-    s_base[tid] = lo;
-  }
-  __syncthreads();
-  // bucket sizes -> exclusive prefix (<= 125 entries, one thread)
-  if (tid < nbk) s_pre[tid] = mysize;
-  __syncthreads();
-  if (tid == 0) {
-    int run = 0;
-    for (int b = 0; b < nbk; ++b) {
-      const int sz = s_pre[b];
-      s_pre[b] = run;
-      run += sz;
-    }
-    s_pre[nbk] = run;
-  }
-  __syncthreads();
-  const int total = s_pre[nbk];
-  for (int idx = tid; idx < total; idx += DEP_NT) {
-    int lo = 0, hi = nbk - 1;  // last b with s_pre[b] <= idx
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_pre[mid] <= idx) lo = mid; else hi = mid - 1;
-    }
-    const float4 r = sorted[s_base[lo] + (idx - s_pre[lo])];
-    for_each_corner(a, r.x, r.y, r.z, r.w, [&](int ix, int iy, int iz, float wt) {
-      const int lx = ix - org[0], ly = iy - org[1], lz = iz - org[2];
-      if ((unsigned)lx < (unsigned)ext[0] && (unsigned)ly < (unsigned)ext[1] && (unsigned)lz < (unsigned)ext[2])
-        atomicAdd(&tilebuf[(lx * tdy + ly) * tdz + lz], wt);
-    });
-  }
-  __syncthreads();
+  // mesh_in may alias mesh_out: each cell is read once and then written by this thread
+  const double inv = aux[1];
   double msum = 0.0;
   for (int i = tid; i < cells; i += DEP_NT) {
-    const int lz = i % tdz, ly = (i / tdz) % tdy, lx = i / (tdz * tdy);
-    if (lx < ext[0] && ly < ext[1] && lz < ext[2]) {
-      const size_t gi = ((size_t)(org[0] + lx) * n + (org[1] + ly)) * n + (org[2] + lz);
-      const float v = tilebuf[i];
-      msum += (double)v;
-      mesh_out[gi] = v;
-    }
+    const float v = FIXED ? (float)((double)(long long)acc_q[i] * inv) : acc_f[i];
+    msum += FIXED ? (double)(long long)acc_q[i] * inv : (double)v;
+    mesh_out[gbase + i] = mesh_in ? mesh_in[gbase + i] + v : v;
   }
-  const double m = block_sum_f64(msum - minit, s_red);
-  if (tid == 0) tile_mass[tile] = m;
+  const double m = block_sum_f64(msum, red, DEP_NT);
+  if (tid == 0) tile_mass[t] = m;
 }
 
 // ------------------------------------------------------------------------------------
 // 5. global-atomic deposit: all particles (direct mode) or the irregular bucket
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(DEP_NT) k_atomic_deposit(ScatterArgs a, const float4* __restrict__ sorted,
+constexpr int ATM_NT = 256;
+
+__global__ void __launch_bounds__(ATM_NT) k_atomic_deposit(ScatterArgs a, const void* __restrict__ sorted,
                                                            const int* __restrict__ range, float* __restrict__ mesh,
                                                            double* __restrict__ stats) {
-  __shared__ double s_red[DEP_NT];
+  __shared__ double red[ATM_NT];
   int64_t lo = 0, hi = a.np;
   if (range) {
     lo = range[0];
@@ -455,7 +279,7 @@ __global__ void __launch_bounds__(DEP_NT) k_atomic_deposit(ScatterArgs a, const 
   for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi; i += (int64_t)gridDim.x * blockDim.x) {
     float px, py, pz, w;
     if (sorted) {
-      const float4 r = sorted[i];
+      const float4 r = load_record(sorted, (size_t)i, a.w != nullptr);
       px = r.x; py = r.y; pz = r.z; w = r.w;
     } else {
       load_particle(a, i, px, py, pz, w);
@@ -466,19 +290,19 @@ __global__ void __launch_bounds__(DEP_NT) k_atomic_deposit(ScatterArgs a, const 
       msum += (double)wt;
     });
   }
-  const double m = block_sum_f64(msum, s_red);
+  const double m = block_sum_f64(msum, red, ATM_NT);
   if (threadIdx.x == 0 && m != 0.0) atomicAdd(&stats[2], m);
 }
 
 // ------------------------------------------------------------------------------------
 // mass finalize: stats[0] = M (fixed-order sum of tile masses + atomic part), stats[1] = n^3 / M
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(DEP_NT) k_mass_finalize(const double* __restrict__ tile_mass, int ntiles,
+__global__ void __launch_bounds__(ATM_NT) k_mass_finalize(const double* __restrict__ tile_mass, int ntiles,
                                                           double* __restrict__ stats, double ncells) {
-  __shared__ double s_red[DEP_NT];
+  __shared__ double red[ATM_NT];
   double s = 0.0;
-  for (int i = threadIdx.x; i < ntiles; i += DEP_NT) s += tile_mass[i];
-  const double m = block_sum_f64(s, s_red) + stats[2];
+  for (int i = threadIdx.x; i < ntiles; i += ATM_NT) s += tile_mass[i];
+  const double m = block_sum_f64(s, red, ATM_NT) + stats[2];
   if (threadIdx.x == 0) {
     stats[0] = m;
     stats[1] = ncells / m;
@@ -488,47 +312,68 @@ __global__ void __launch_bounds__(DEP_NT) k_mass_finalize(const double* __restri
 // ------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------
-int make_tile_geom(int n, int stencil, const int* tile, TileGeom* g, bool* direct) {
+constexpr size_t LDS_MAX = 160 * 1024;
+constexpr int CHUNK = 65535;
+
+static int largest_divisor_le(int n, int cap) {
+  for (int d = cap < n ? cap : n; d >= 2; --d)
+    if (n % d == 0) return d;
+  return n;
+}
+
+int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g, bool* direct) {
   *direct = false;
-  int td[3] = {16, 16, 32};
+  *g = TileGeom{};
+  int ty = 0;
   if (tile) {
     if (tile[0] == 0) {
       *direct = true;
-      *g = TileGeom{};
       return PK_OK;
     }
-    td[0] = tile[0];
-    td[1] = tile[1];
-    td[2] = tile[2];
+    ty = tile[1];
+    if (tile[0] != 1 || (tile[2] != 0 && tile[2] != n))
+      return set_error(PK_E_INVALID, "sorted deposit tiles are (1, ty, n) row bundles, got (%d, %d, %d)", tile[0],
+                       tile[1], tile[2]);
+    if (ty < 1 || ty > n || n % ty != 0) return set_error(PK_E_INVALID, "rows per tile %d must divide n=%d", ty, n);
+    if (ty < 2 && n >= 2) return set_error(PK_E_INVALID, "rows per tile must be >= 2");
+  } else {
+    ty = largest_divisor_le(n, 32);
   }
-  long cells = 1;
-  for (int ax = 0; ax < 3; ++ax) {
-    if (td[ax] > n) td[ax] = n;
-    if (td[ax] < 2) return set_error(PK_E_INVALID, "tile dim %d < 2", td[ax]);
-    const int rem = n % td[ax];
-    if (rem == 1) return set_error(PK_E_INVALID, "tile dim %d leaves a 1-cell tile for n=%d", td[ax], n);
-    g->td[ax] = td[ax];
-    g->nt[ax] = (n + td[ax] - 1) / td[ax];
-    cells *= td[ax];
+  g->ty = ty;
+  g->nyt = n / ty;
+  g->ncls = stencil == PK_STENCIL_TSC ? 3 : 2;
+  const long ntiles = (long)n * g->nyt;
+  const long nkeys = ntiles * g->ncls + 1;
+  const size_t dest_bytes = (size_t)nkeys * 4;   // the place kernel's per-key write positions
+  const size_t rows_bytes = (size_t)ty * n * sizeof(float);
+  if (dest_bytes > LDS_MAX - 8192 || rows_bytes > LDS_MAX - 8192) {
+    if (tile) return set_error(PK_E_INVALID, "tile of %d rows x %d needs too much LDS", ty, n);
+    *direct = true;   // very large meshes with TSC: global atomics
+    return PK_OK;
   }
-  if (cells * 4 > 65536) return set_error(PK_E_INVALID, "tile of %ld cells exceeds 64 KiB of LDS", cells);
-  g->ncls = stencil == PK_STENCIL_TSC ? 27 : 8;
-  g->ntiles = g->nt[0] * g->nt[1] * g->nt[2];
-  const long nkeys = (long)g->ntiles * g->ncls + 1;
-  if (nkeys > (long)SCAN_BLOCK * SCAN_BLOCK) return set_error(PK_E_INVALID, "too many tile buckets (%ld)", nkeys);
+  g->ntiles = (int)ntiles;
   g->nkeys = (int)nkeys;
+  g->fixed = (size_t)ty * n * 8 <= FIXED_LDS_MAX ? 1 : 0;
+  g->chunk = CHUNK;
+  g->nblk = (int)((np + CHUNK - 1) / CHUNK);
+  if (g->nblk < 1) g->nblk = 1;
+  if ((long)g->nkeys > (long)SCAN_BLOCK * SCAN_BLOCK || g->nblk > SCAN_BLOCK * SCAN_BLOCK)
+    return set_error(PK_E_INVALID, "too many buckets");
   return PK_OK;
 }
 
 namespace {
 struct ScatterWS {
-  int* counts;
-  int* offsets;
-  int* cursor;
-  int* bsum;
-  float4* sorted;
-  double* tile_mass;
   double* stats;
+  unsigned* mat;
+  int* totals;
+  int* key_start;
+  int* bsum;
+  double* wsum;
+  double* wmax;
+  double* aux;
+  void* sorted;
+  double* tile_mass;
   size_t bytes;
 };
 
@@ -543,11 +388,14 @@ ScatterWS carve(const TileGeom& g, bool direct, int64_t np, void* base) {
   };
   w.stats = static_cast<double*>(take(PK_STATS_LEN * sizeof(double)));
   if (!direct) {
-    w.counts = static_cast<int*>(take((size_t)g.nkeys * sizeof(int)));
-    w.offsets = static_cast<int*>(take(((size_t)g.nkeys + 1) * sizeof(int)));
-    w.cursor = static_cast<int*>(take((size_t)g.nkeys * sizeof(int)));
+    w.mat = static_cast<unsigned*>(take((size_t)g.nblk * g.nkeys * sizeof(unsigned)));
+    w.totals = static_cast<int*>(take((size_t)g.nkeys * sizeof(int)));
+    w.key_start = static_cast<int*>(take(((size_t)g.nkeys + 1) * sizeof(int)));
     w.bsum = static_cast<int*>(take((size_t)SCAN_BLOCK * sizeof(int)));
-    w.sorted = static_cast<float4*>(take((size_t)(np > 0 ? np : 1) * sizeof(float4)));
+    w.wsum = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
+    w.wmax = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
+    w.aux = static_cast<double*>(take(AUX_LEN * sizeof(double)));
+    w.sorted = take((size_t)(np > 0 ? np : 1) * sizeof(float4));
     w.tile_mass = static_cast<double*>(take((size_t)g.ntiles * sizeof(double)));
   }
   w.bytes = off;
@@ -555,10 +403,18 @@ ScatterWS carve(const TileGeom& g, bool direct, int64_t np, void* base) {
 }
 
 int grid_for(int64_t items, int nt, int cap) {
-  int64_t g = (items + nt - 1) / nt;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return (int)g;
+  int64_t gsz = (items + nt - 1) / nt;
+  if (gsz < 1) gsz = 1;
+  if (gsz > cap) gsz = cap;
+  return (int)gsz;
+}
+
+template <typename K>
+int allow_lds(K kernel, size_t bytes) {
+  if (bytes <= 65536) return PK_OK;
+  return check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+                   "LDS attribute");
 }
 }  // namespace
 
@@ -566,75 +422,102 @@ size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np) {
   return carve(g, direct, np, nullptr).bytes;
 }
 
+int mass_finalize(const double* tile_mass, int ntiles, double* stats, int n, hipStream_t s) {
+  StageTimer t(ST_MASS, s);
+  hipLaunchKernelGGL(k_mass_finalize, dim3(1), dim3(ATM_NT), 0, s, tile_mass, ntiles, stats,
+                     (double)n * (double)n * (double)n);
+  return check_hip(hipGetLastError(), "mass finalize launch");
+}
+
 int scatter_run(const ScatterArgs& a, const TileGeom& g, bool direct, const float* mesh_in, float* mesh_out,
-                double* stats, void* ws, size_t ws_bytes, hipStream_t s) {
+                double* stats, void* ws, size_t ws_bytes, hipStream_t s, const void** sorted_out,
+                const int** key_start_out, double** tile_mass_out, const double** aux_out) {
   ScatterWS w = carve(g, direct, a.np, ws);
   if (ws_bytes < w.bytes) return set_error(PK_E_WORKSPACE, "scatter workspace %zu < %zu bytes", ws_bytes, w.bytes);
   double* st = stats ? stats : w.stats;
-  const int dev_cap = 65535;
   const int64_t ncell = (int64_t)a.n * a.n * a.n;
   {
     StageTimer t(ST_FILL, s);
     hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(PK_STATS_LEN), 0, s, st, PK_STATS_LEN, 0.0);
-    if (!direct) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(g.nkeys, 256, 4096)), dim3(256), 0, s, w.counts,
-                                    (int64_t)g.nkeys, 0);
-    else hipLaunchKernelGGL(k_copy_or_zero, dim3(grid_for(ncell, 256, 16384)), dim3(256), 0, s, mesh_in, mesh_out, ncell);
+    if (direct) hipLaunchKernelGGL(k_copy_or_zero, dim3(grid_for(ncell, 256, 16384)), dim3(256), 0, s, mesh_in,
+                                   mesh_out, ncell);
     int e = check_hip(hipGetLastError(), "scatter fill launch");
     if (e) return e;
   }
   if (direct) {
+    if (!mesh_out) return set_error(PK_E_INVALID, "direct mode needs a mesh");
     if (a.np > 0) {
       StageTimer t(ST_ATOMIC, s);
-      hipLaunchKernelGGL(k_atomic_deposit, dim3(grid_for(a.np, DEP_NT, dev_cap)), dim3(DEP_NT), 0, s, a,
-                         (const float4*)nullptr, (const int*)nullptr, mesh_out, st);
+      hipLaunchKernelGGL(k_atomic_deposit, dim3(grid_for(a.np, ATM_NT, 65535)), dim3(ATM_NT), 0, s, a,
+                         (const void*)nullptr, (const int*)nullptr, mesh_out, st);
       int e = check_hip(hipGetLastError(), "atomic deposit launch");
       if (e) return e;
     }
-    StageTimer t(ST_MASS, s);
-    hipLaunchKernelGGL(k_mass_finalize, dim3(1), dim3(DEP_NT), 0, s, (const double*)nullptr, 0, st, (double)ncell);
-    return check_hip(hipGetLastError(), "mass finalize launch");
+    return mass_finalize(nullptr, 0, st, a.n, s);
   }
-  const int pgrid = grid_for(a.np, 256, 8192);
-  if (a.np > 0) {
+  const size_t hist_bytes = (size_t)((g.nkeys + 1) / 2) * 4;
+  {
     StageTimer t(ST_SORT_COUNT, s);
-    hipLaunchKernelGGL(k_sort_count, dim3(pgrid), dim3(256), 0, s, a, g, w.counts);
-    int e = check_hip(hipGetLastError(), "sort count launch");
+    int e = allow_lds(k_bucket_count, hist_bytes);
+    if (e) return e;
+    hipLaunchKernelGGL(k_bucket_count, dim3(g.nblk), dim3(CNT_NT), hist_bytes, s, a, g, w.mat, w.wsum, w.wmax);
+    e = check_hip(hipGetLastError(), "bucket count launch");
     if (e) return e;
   }
   {
     StageTimer t(ST_SCAN, s);
-    const int nblk = (g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    hipLaunchKernelGGL(k_scan_local, dim3(nblk), dim3(SCAN_NT), 0, s, w.counts, w.offsets, w.bsum, g.nkeys);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_NT), 0, s, w.bsum, nblk, w.offsets + g.nkeys);
-    hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(SCAN_NT), 0, s, w.offsets, w.cursor, w.bsum, g.nkeys);
+    hipLaunchKernelGGL(k_col_scan, dim3((g.nkeys + 255) / 256), dim3(256), 0, s, w.mat, g.nblk, g.nkeys, w.totals);
+    const int nb2 = (g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    hipLaunchKernelGGL(k_scan_local, dim3(nb2), dim3(SCAN_NT), 0, s, w.totals, w.key_start, w.bsum, g.nkeys);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_NT), 0, s, w.bsum, nb2, w.key_start + g.nkeys,
+                       (const double*)w.wsum, (const double*)w.wmax, g.nblk, st, w.aux, (double)ncell);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb2), dim3(SCAN_NT), 0, s, w.key_start, (const int*)w.bsum, g.nkeys);
     int e = check_hip(hipGetLastError(), "scan launch");
     if (e) return e;
   }
-  if (a.np > 0) {
+  {
     StageTimer t(ST_SORT_PLACE, s);
-    hipLaunchKernelGGL(k_sort_place, dim3(pgrid), dim3(256), 0, s, a, g, w.cursor, w.sorted);
-    int e = check_hip(hipGetLastError(), "sort place launch");
+    const size_t dest_bytes = (size_t)g.nkeys * 4;
+    int e = allow_lds(k_bucket_place, dest_bytes);
     if (e) return e;
+    hipLaunchKernelGGL(k_bucket_place, dim3(g.nblk), dim3(CNT_NT), dest_bytes, s, a, g, (const unsigned*)w.mat,
+                       (const int*)w.key_start, w.sorted);
+    e = check_hip(hipGetLastError(), "bucket place launch");
+    if (e) return e;
+  }
+  if (!mesh_out) {  // the caller deposits (fused with the z pass)
+    if (sorted_out) *sorted_out = w.sorted;
+    if (key_start_out) *key_start_out = w.key_start;
+    if (tile_mass_out) *tile_mass_out = w.tile_mass;
+    if (aux_out) *aux_out = w.aux;
+    return PK_OK;
   }
   {
     StageTimer t(ST_DEPOSIT, s);
-    const size_t lds = (size_t)g.td[0] * g.td[1] * g.td[2] * sizeof(float);
-    hipLaunchKernelGGL(k_deposit_tiles, dim3(g.ntiles), dim3(DEP_NT), lds, s, a, g, w.offsets,
-                       (const float4*)w.sorted, mesh_in, mesh_out, w.tile_mass);
-    int e = check_hip(hipGetLastError(), "tile deposit launch");
+    const size_t lds = (size_t)g.ty * a.n * (g.fixed ? 8 : 4);
+    int e;
+    if (g.fixed) {
+      e = allow_lds(k_rows_deposit<true>, lds);
+      if (e) return e;
+      hipLaunchKernelGGL(k_rows_deposit<true>, dim3(g.ntiles), dim3(DEP_NT), lds, s, a, g, (const int*)w.key_start,
+                         (const void*)w.sorted, (const double*)w.aux, mesh_in, mesh_out, w.tile_mass);
+    } else {
+      e = allow_lds(k_rows_deposit<false>, lds);
+      if (e) return e;
+      hipLaunchKernelGGL(k_rows_deposit<false>, dim3(g.ntiles), dim3(DEP_NT), lds, s, a, g, (const int*)w.key_start,
+                         (const void*)w.sorted, (const double*)w.aux, mesh_in, mesh_out, w.tile_mass);
+    }
+    e = check_hip(hipGetLastError(), "row deposit launch");
     if (e) return e;
   }
   if (a.stencil == PK_STENCIL_CIC_REF && a.np > 0) {
     StageTimer t(ST_ATOMIC, s);
-    hipLaunchKernelGGL(k_atomic_deposit, dim3(64), dim3(DEP_NT), 0, s, a, (const float4*)w.sorted,
-                       (const int*)(w.offsets + g.nkeys - 1), mesh_out, st);
+    hipLaunchKernelGGL(k_atomic_deposit, dim3(64), dim3(ATM_NT), 0, s, a, (const void*)w.sorted,
+                       (const int*)(w.key_start + g.nkeys - 1), mesh_out, st);
     int e = check_hip(hipGetLastError(), "irregular deposit launch");
     if (e) return e;
   }
-  StageTimer t(ST_MASS, s);
-  hipLaunchKernelGGL(k_mass_finalize, dim3(1), dim3(DEP_NT), 0, s, (const double*)w.tile_mass, g.ntiles, st,
-                     (double)ncell);
-  return check_hip(hipGetLastError(), "mass finalize launch");
+  return mass_finalize(w.tile_mass, g.ntiles, st, a.n, s);
 }
 
 }  // namespace pk

@@ -95,7 +95,8 @@ def assign(x, y, z, w=None, *, box_size, n_bins, mas="CIC_REF", wrap=True, xmin=
 
     mas: 'CIC_REF' (reference stencil, cic_mas_vec), 'CIC' (textbook, mass-conserving,
     periodic) or 'TSC'.  delta: optional accumulation base (None = zeros).
-    tile: (tx, ty, tz) LDS tile of the sorted deposit, 'direct' for global atomics, or None.
+    tile: (1, ty, 0) row-bundle tile of the sorted deposit (ty rows of full z extent, ty | n),
+          'direct' for global atomics, or None (ty = largest divisor of n <= 32).
     stats: optional float64 device tensor of 4 entries receiving [mass, n^3/mass, -, -].
     """
     mas = mas.upper()

@@ -16,6 +16,7 @@ from oracle import ref_fp64 as R
 pytestmark = pytest.mark.gpu
 F32 = np.float32
 PK_TOL = 1e-5
+PK_TOL_CORNER = 1e-4
 
 
 def catalogue(rng, npart, box, clumps=True, edges=True):
@@ -72,8 +73,12 @@ def pk_check(pk, nm, ex, o_pk, o_extra, box, n, shot=0.0, sum_abs=None):
         assert abs(np.sqrt(max(S[slot0, 0], 0)) - np.sqrt(max(So, 0))) <= 1e-12 * (sum_abs or 1.0)
     scale = o_extra["scale"][check]
     err = np.abs(P[check] - o[check]) / np.maximum(scale, 1e-300)
-    worst = float(np.nanmax(err)) if err.size else 0.0
-    assert worst <= PK_TOL, (worst, np.unravel_index(np.nanargmax(err), err.shape))
+    # bins beyond the Nyquist sphere (k > k_N = n/2 k_F) see the float32 mesh round-off
+    # amplified by the window deconvolution (up to (pi/2)^(4p) per axis): 1e-4 there
+    kbin = o_extra["kmean"][check] / (2 * np.pi / box)
+    tol = np.where(np.isfinite(kbin) & (kbin > n / 2), PK_TOL_CORNER, PK_TOL)[:, None]
+    worst = float(np.nanmax(err / tol)) if err.size else 0.0
+    assert worst <= 1.0, (worst, np.unravel_index(np.nanargmax(err / tol), err.shape))
     # float32 outputs = rounding of (S / N * norm * (2l+1), minus the shot noise on l = 0)
     want32 = (P - np.array([shot, 0.0, 0.0])).astype(F32)
     np.testing.assert_allclose(pk.cpu().numpy()[~empty], want32[~empty], rtol=2e-7, atol=0)
@@ -84,11 +89,9 @@ def pk_check(pk, nm, ex, o_pk, o_extra, box, n, shot=0.0, sum_abs=None):
 # mass assignment
 # ----------------------------------------------------------------------------------------
 @pytest.mark.parametrize("n", [24, 32])
-@pytest.mark.parametrize("tile", [None, (4, 4, 8), (8, 6, 10), "direct"])
+@pytest.mark.parametrize("tile", [None, (1, 4, 0), (1, 2, 0), "direct"])
 @pytest.mark.parametrize("fn,wrap", [("vec", True), ("vec", False), ("scan", True), ("scan", False)])
 def test_reference_stencil_parity(gpu, n, tile, fn, wrap):
-    if tile == (8, 6, 10) and n == 32:
-        tile = (8, 6, 12)                     # leaves an 8-cell last tile on z
     rng = np.random.default_rng(100 + n)
     box = 200.0
     p = ref_catalogue(rng, 5000, box, n)
@@ -126,7 +129,8 @@ def test_dropin_signatures(gpu):
 
 
 @pytest.mark.parametrize("stencil", ["CIC", "TSC"])
-@pytest.mark.parametrize("n,tile", [(16, None), (24, (4, 4, 8)), (32, (8, 8, 16)), (48, "direct"), (32, (2, 2, 2))])
+@pytest.mark.parametrize("n,tile", [(16, None), (24, (1, 4, 0)), (32, (1, 8, 32)), (48, "direct"), (32, (1, 2, 0)),
+                                    (12, (1, 6, 0))])
 def test_cic_tsc_parity(gpu, stencil, n, tile):
     rng = np.random.default_rng(7 + n)
     box = 300.0
@@ -233,8 +237,10 @@ def test_powspec_vec_fundamental_parity(gpu, n):
 # whole path
 # ----------------------------------------------------------------------------------------
 @pytest.mark.parametrize("stencil", ["CIC_REF", "CIC", "TSC"])
-@pytest.mark.parametrize("n,kind", [(32, "kf_half"), (48, "fundamental"), (64, "kf_integer_with_zero")])
-def test_estimate_parity(gpu, stencil, n, kind):
+@pytest.mark.parametrize("n,kind,tile", [(32, "kf_half", None), (48, "fundamental", None),
+                                         (64, "kf_integer_with_zero", None), (32, "kf_half", "direct"),
+                                         (24, "fundamental", (1, 6, 0)), (16, "kf_half", None)])
+def test_estimate_parity(gpu, stencil, n, kind, tile):
     rng = np.random.default_rng(n + len(stencil))
     box = 1000.0
     p = catalogue(rng, 60000, box)
@@ -243,7 +249,7 @@ def test_estimate_parity(gpu, stencil, n, kind):
     shot = box ** 3 / len(p)
     t = torch.as_tensor(p, device=gpu)
     k3, pk, nm, ex = correlations.estimate(t[:, 0], t[:, 1], t[:, 2], box_size=box, n_bins=n, k_edges=k_edges,
-                                           mas=stencil, shot_noise=shot, full=True)
+                                           mas=stencil, shot_noise=shot, full=True, tile=tile)
     window = 3 if stencil == "TSC" else 2
     res = R.estimate(*p.T, None, box, n, k_edges, stencil=stencil, p=window, shot_noise=shot)
     ok3, opk, onm, oex = res[:4]
@@ -300,3 +306,18 @@ def test_full_size_counts_and_determinism(gpu):
     # be positive, finite and of order L^3 / N_p everywhere
     P0 = outs[0][1][:, 0].cpu().numpy()
     assert np.all(np.isfinite(P0)) and np.all(P0 > 0.3 * box ** 3 / npart) and np.all(P0 < 30 * box ** 3 / npart)
+
+
+def test_estimate_reference_stencil_with_outside_particles(gpu):
+    # irregular particles of the reference stencil (outside [0, L)) in the fused path
+    rng = np.random.default_rng(77)
+    n, box = 32, 400.0
+    p = ref_catalogue(rng, 30000, box, n)
+    kF = 2 * np.pi / box
+    edges = kF * (0.5 + np.arange(n // 2 + 1))
+    t = torch.as_tensor(p, device=gpu)
+    for tile in (None, "direct"):
+        k3, pk, nm, ex = correlations.estimate(t[:, 0], t[:, 1], t[:, 2], box_size=box, n_bins=n, k_edges=edges,
+                                               full=True, tile=tile)
+        ok3, opk, onm, oex = R.estimate(*p.T, None, box, n, edges, stencil="CIC_REF")[:4]
+        pk_check(pk, nm, ex, opk, oex, box, n)
