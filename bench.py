@@ -79,17 +79,89 @@ def make_catalogue(n, box, nbar, device, seed=2002):
     return pos
 
 
-def algorithmic_bytes(n, npart):
-    """Minimum HBM bytes each stage must move (per launch)."""
+def algorithmic_bytes(n, npart, fused, rec=12):
+    """Minimum HBM bytes each stage must move per launch (rec = bytes per sorted record:
+    12 for unit weights, 16 with weights)."""
     half = 8 * n * n * (n // 2 + 1)
-    return {
+    ab = {
         "sort_count": 12 * npart,                  # read positions
-        "sort_place": 12 * npart + 16 * npart,     # read positions, write sorted records
-        "deposit": 16 * npart + 4 * n ** 3,        # read records, write the mesh once
-        "zpass": 4 * n ** 3 + half,                # read mesh, write half spectrum
+        "sort_place": 12 * npart + rec * npart,    # read positions, write sorted records
         "ypass": 2 * half,                         # in place
         "xpass_bin": half,                         # read; only nb-sized sums leave
     }
+    if fused:   # deposit fused with the z pass: read records, write the half spectrum
+        ab["deposit"] = rec * npart + half
+    else:
+        ab["deposit"] = rec * npart + 4 * n ** 3
+        ab["zpass"] = 4 * n ** 3 + half
+    return ab
+
+
+# stage -> kernel-name fragment in rocprofv3 output
+STAGE_KERNELS = {
+    "sort_count": "k_bucket_count", "sort_place": "k_bucket_place", "ypass": "k_ypass<",
+    "xpass_bin": "k_xpass_bin<", "zpass": "k_zpass<",
+}
+
+
+def stage_kernel(stage, n, fused):
+    if stage == "deposit":
+        return f"k_fused_deposit_zpass<{n}>" if fused else "k_rows_deposit"
+    return STAGE_KERNELS.get(stage, stage)
+
+
+def read_traffic(path):
+    """HBM bytes per dispatch from a traffic summary written by --pmc-summary:
+    lines 'kernel<TAB>fetch_bytes<TAB>write_bytes' after the header lines starting with '#'."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        k, f, w = line.rstrip("\n").split("\t")[:3]
+        out[k] = (float(f), float(w))
+    return out
+
+
+def pmc_summary(fetch_csv, write_csv, out_path, command):
+    """Per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two rocprofv3
+    --pmc runs, calibrated with the 1 GiB device copy of --calib-copy (the first copy
+    dispatches: 1 GiB read, 1 GiB written)."""
+    import collections
+    import csv
+
+    def load(path, ctr):
+        agg = collections.defaultdict(list)
+        copies = []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != ctr:
+                continue
+            name = r["Kernel_Name"]
+            v = float(r["Counter_Value"]) * 1024.0
+            if "copyBuffer" in name and int(r["Grid_Size"]) == 131072 and len(copies) < 3:
+                copies.append(v)
+            agg[name].append(v)
+        return agg, copies
+
+    fa, fc = load(fetch_csv, "FETCH_SIZE")
+    wa, wc = load(write_csv, "WRITE_SIZE")
+    gib = float(1 << 30)
+    ff = gib / (sum(fc) / len(fc))
+    fw = gib / (sum(wc) / len(wc))
+    lines = [f"# HBM traffic per dispatch (bytes), rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE",
+             f"# raw counters: {command}",
+             f"# calibration: 1 GiB device copy reads FETCH_SIZE x {ff:.3f}, writes WRITE_SIZE x {fw:.3f}",
+             "# kernel\tfetch_bytes\twrite_bytes\tdispatches"]
+    for name in sorted(fa):
+        if not name.startswith("void pk::") and not name.startswith("pk::"):
+            continue
+        f = sum(fa[name]) / len(fa[name]) * ff
+        w = sum(wa.get(name, [0.0])) / max(1, len(wa.get(name, []))) * fw
+        short = name.split("(")[0].replace("void ", "").replace("pk::", "")
+        lines.append(f"{short}\t{f:.0f}\t{w:.0f}\t{len(fa[name])}")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def cpu_baseline(pos_host, n, box, edges, shot):
@@ -113,7 +185,15 @@ def main():
     ap.add_argument("--tile", type=str, default=None, help="1,ty,0 | direct")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (HBM counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the calibrated HBM traffic summary of two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--pmc-command", default="", help="command line recorded in the --pmc-summary header")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_cic512_2e7.txt"),
+                    help="traffic summary read for roofline.traffic")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.traffic, args.pmc_command)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the path's "
               "multi-GPU sharding is a design, not a measured configuration", file=sys.stderr)
@@ -176,7 +256,8 @@ def main():
     log(f"P_0 first bins: {pk[:4, 0]}")
     roof = None
     if stages:
-        ab = algorithmic_bytes(N_MESH, npart)
+        fused = "zpass" not in stages
+        ab = algorithmic_bytes(N_MESH, npart, fused)
         dom = max(stages, key=lambda s: stages[s]["avg_ms"])
         for s, v in stages.items():
             if s in ab:
@@ -184,9 +265,14 @@ def main():
         log("stages: " + json.dumps(stages))
         if dom in ab:
             ach = ab[dom] / (stages[dom]["avg_ms"] * 1e-3) / 1e9
-            roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                    "algorithmic_bytes": ab[dom], "avg_ms": stages[dom]["avg_ms"]}
+            kname = stage_kernel(dom, N_MESH, fused)
+            traffic = None
+            for k, (f, w) in read_traffic(args.traffic).items():
+                if kname in k:
+                    traffic = round(f + w)
+            roof = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algorithmic_bytes": ab[dom], "avg_ms": round(stages[dom]["avg_ms"], 5)}
         path_bytes = 12 * npart + 2 * 4 * N_MESH ** 3 + 4 * 8 * N_MESH * N_MESH * (N_MESH // 2 + 1)
         roof_path = path_bytes / (ms * 1e-3) / 1e9
         log(f"whole-path algorithmic bytes {path_bytes / 1e9:.3f} GB -> {roof_path:.0f} GB/s "
@@ -196,7 +282,7 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(pos.cpu().numpy(), N_MESH, BOX, edges, shot)
     out = {
-        "metric": "P_l(k) estimates/sec at 512^3, 2e7 particles",
+        "metric": "P_l(k) estimates/sec at 512^3, 2e7 particles; batched/served vs plain; p99 ms",
         "value": round(1e3 / ms, 3),
         "unit": "estimates/s",
         "n_gpus": 1,
