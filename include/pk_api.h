@@ -140,6 +140,52 @@ int pk_estimate(pk_plan* plan,
                 double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
                 double* stats, void* workspace, size_t workspace_bytes, void* stream);
 
+/* --- batches: many catalogues of one mesh side per launch --------------------------
+ * No reference counterpart: the reference estimates one catalogue per call.  A batch
+ * runs up to max_jobs estimates (each as pk_estimate: its own particles, stencil, window,
+ * bins, shot noise and outputs) with one launch per stage for all of them: blockIdx.y
+ * selects the job in a device-resident job table.  Every job's scratch (spectrum, sort
+ * buffers, bin partials) is a fixed slot of the caller's workspace sized for max_np
+ * particles and max_nb bins, so a batch can be refilled (pk_batch_set_jobs) and re-run
+ * without allocation.  With PK_BATCH_GRAPH the launch sequence is captured into a HIP
+ * graph on the first run of each job-count shape and replayed afterwards.
+ *   - all jobs use the sorted + fused assignment path (tile == NULL: default rows per
+ *     tile; tile = (1, ty, 0) with a compiled fused variant); pk_batch_create fails with
+ *     PK_E_UNSUPPORTED when n has no such path (n >= 768);
+ *   - pk_batch_set_jobs copies the host job array into pinned memory and enqueues its
+ *     upload on `stream`; the particle and output pointers must stay valid until the
+ *     runs that use them are complete;
+ *   - a batch is used by one stream at a time; destroy it only after its work is done.
+ */
+typedef struct pk_job {
+  const float* x; int64_t sx;
+  const float* y; int64_t sy;
+  const float* z; int64_t sz;
+  const float* w; int64_t sw;            /* w may be NULL (unit weights) */
+  int64_t np;
+  float xmin, ymin, zmin;
+  double box_size;
+  int stencil, wrap_mode, window_p, flags;
+  const int32_t* thr; const int32_t* jstart; int nb;
+  double shot_noise;
+  double* sums; int64_t* counts; float* pk; float* nmodes; double* kmean;  /* as pk_estimate */
+  double* stats;                          /* nullable: PK_STATS_LEN doubles */
+} pk_job;
+
+typedef struct pk_batch pk_batch;
+
+#define PK_BATCH_GRAPH 1
+
+int pk_batch_workspace_size(pk_plan* plan, int max_jobs, int64_t max_np, int max_nb, const int* tile,
+                            size_t* bytes);
+int pk_batch_create(pk_plan* plan, int max_jobs, int64_t max_np, int max_nb, const int* tile, int flags,
+                    void* workspace, size_t workspace_bytes, pk_batch** batch);
+int pk_batch_set_jobs(pk_batch* batch, int njobs, const pk_job* jobs, void* stream);
+int pk_batch_run(pk_batch* batch, void* stream);
+/* resolved tile (1, ty, n) of the batch's fused deposit; host int[3] */
+int pk_batch_tile(pk_batch* batch, int* tile);
+int pk_batch_destroy(pk_batch* batch);
+
 /* --- per-stage timing (bench / profiling; no reference counterpart) ----------------
  * When enabled, each kernel stage is bracketed by HIP events on its own stream.
  * pk_timing_read synchronises those events, returns total ms and launch counts per stage

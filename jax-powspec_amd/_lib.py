@@ -33,6 +33,21 @@ _f = C.c_float
 _d = C.c_double
 _sz = C.c_size_t
 _psz = C.POINTER(C.c_size_t)
+BATCH_GRAPH = 1
+
+
+class PkJob(C.Structure):
+    """struct pk_job of pk_api.h (one catalogue of a batch)."""
+    _fields_ = [
+        ("x", _vp), ("sx", _i64), ("y", _vp), ("sy", _i64), ("z", _vp), ("sz", _i64),
+        ("w", _vp), ("sw", _i64), ("np", _i64),
+        ("xmin", _f), ("ymin", _f), ("zmin", _f), ("box_size", _d),
+        ("stencil", _i), ("wrap_mode", _i), ("window_p", _i), ("flags", _i),
+        ("thr", _vp), ("jstart", _vp), ("nb", _i), ("shot_noise", _d),
+        ("sums", _vp), ("counts", _vp), ("pk", _vp), ("nmodes", _vp), ("kmean", _vp), ("stats", _vp),
+    ]
+
+
 SIGNATURES = [
     ("pk_version", _i, []),
     ("pk_last_error", C.c_char_p, []),
@@ -49,6 +64,12 @@ SIGNATURES = [
     ("pk_estimate_workspace_size", _i, [_vp, _i64, _i, _vp, _i, _psz]),
     ("pk_estimate", _i, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _f, _f, _f, _d, _i, _i, _vp,
                          _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    ("pk_batch_workspace_size", _i, [_vp, _i, _i64, _i, _vp, _psz]),
+    ("pk_batch_create", _i, [_vp, _i, _i64, _i, _vp, _i, _vp, _sz, C.POINTER(_vp)]),
+    ("pk_batch_set_jobs", _i, [_vp, _i, _vp, _vp]),
+    ("pk_batch_run", _i, [_vp, _vp]),
+    ("pk_batch_tile", _i, [_vp, _vp]),
+    ("pk_batch_destroy", _i, [_vp]),
     ("pk_timing_enable", _i, [_i]),
     ("pk_timing_read", _i, [_vp, _vp, _i]),
     ("pk_stage_name", C.c_char_p, [_i]),

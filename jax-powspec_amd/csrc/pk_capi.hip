@@ -59,8 +59,11 @@ const char* g_stage_names[PK_NUM_STAGES] = {"sort_count", "scan",   "sort_place"
                                             "xpass_bin",  "finalize", "xpass_plain", "fill"};
 }  // namespace
 
+void timing_suspend(bool on);
+bool timing_suspended();
+
 void timing_begin(int stage, hipStream_t s) {
-  if (!g_timing) return;
+  if (!g_timing || timing_suspended()) return;
   hipEvent_t e;
   if (hipEventCreate(&e) != hipSuccess) return;
   (void)hipEventRecord(e, s);
@@ -68,7 +71,7 @@ void timing_begin(int stage, hipStream_t s) {
 }
 
 void timing_end(int stage, hipStream_t s) {
-  if (!g_timing || !g_open[stage]) return;
+  if (!g_timing || timing_suspended() || !g_open[stage]) return;
   hipEvent_t e;
   if (hipEventCreate(&e) != hipSuccess) return;
   (void)hipEventRecord(e, s);
@@ -165,22 +168,36 @@ int check_bins(const int32_t* thr, const int32_t* jstart, int nb, const float* p
   return PK_OK;
 }
 
-// y pass, x pass + binning and finalize on a spectrum whose z pass is done
-int finish_powspec(pk_plan* p, float2* spec, const int32_t* thr, const int32_t* jstart, int nb, int window_p,
-                   int flags, double box, double shot, const double* dc_part, const double* scale_stats,
-                   double* partials, double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
-                   hipStream_t s) {
+// binning fields of a job (everything the y pass, x pass + bin and finalize read)
+void set_bin_fields(Job& j, const pk_plan* p, float2* spec, const int32_t* thr, const int32_t* jstart, int nb,
+                    int window_p, int flags, double box, double shot, double* sums, int64_t* counts, float* pk,
+                    float* nmodes, double* kmean) {
   const int n = p->n;
-  int e = fft_ypass(n, spec, p->tw, s);
-  if (e) return e;
-  const int G = bin_grid(n, nb);
-  const int jmax = (int)floor(sqrt(3.0 * (n / 2) * (n / 2))) + 1;
-  e = fft_xpass_bin(n, spec, p->tw, p->wtab + (size_t)window_p * n, thr, jstart, nb, jmax, flags, partials, G, s);
-  if (e) return e;
+  j.spec = spec;
+  j.wtab = p->wtab + (size_t)window_p * n;
+  j.thr = thr;
+  j.jst = jstart;
+  j.nb = nb;
+  j.flags = flags;
   const double h = box / ((double)n * (double)n);
-  const double norm = h * h * h;  // (box / n^2)^3, src/correlations.py:50-52
-  return bin_finalize(partials, G, nb, thr, dc_part, zpass_grid(n), scale_stats, norm, shot, sums, counts, pk,
-                      nmodes, (flags & PK_FLAG_KMEAN) ? kmean : nullptr, s);
+  j.norm = h * h * h;  // (box / n^2)^3, src/correlations.py:50-52
+  j.shot = shot;
+  j.sums = sums;
+  j.counts = counts;
+  j.pk = pk;
+  j.nmodes = nmodes;
+  j.kmean = kmean;
+}
+
+// y pass, x pass + binning and finalize on a spectrum whose z pass is done
+int finish_powspec(pk_plan* p, const Job& j, hipStream_t s) {
+  const int n = p->n;
+  int e = fft_ypass(n, j, p->tw, s);
+  if (e) return e;
+  const int G = bin_grid(n, j.nb);
+  e = fft_xpass_bin(n, j, p->tw, G, s);
+  if (e) return e;
+  return bin_finalize(j, G, s);
 }
 
 double* dc_partials(void* ws, int n, int nb) {
@@ -196,14 +213,18 @@ int run_powspec(pk_plan* p, const float* mesh, const double* stats, const int32_
   const size_t need = powspec_ws_bytes(n, nb);
   if (ws_bytes < need) return set_error(PK_E_WORKSPACE, "powspec workspace %zu < %zu bytes", ws_bytes, need);
   if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
-  double* partials = static_cast<double*>(ws);
-  double* dc_part = dc_partials(ws, n, nb);
+  Job j{};
+  j.active = 1;
+  set_bin_fields(j, p, spec, thr, jstart, nb, window_p, flags, box, shot, sums, counts, pk, nmodes, kmean);
+  j.partials = static_cast<double*>(ws);
   // with the contrast applied (ZERO_DC) the k = 0 mode is exactly 0 and no DC sum is needed
   const bool keep_dc = (flags & PK_FLAG_ZERO_DC) == 0;
-  int e = fft_zpass(n, mesh, spec, p->tw, stats, keep_dc ? dc_part : nullptr, s);
+  double* dc_part = keep_dc ? dc_partials(ws, n, nb) : nullptr;
+  j.dc_part = dc_part;
+  j.Gz = zpass_grid(n);
+  int e = fft_zpass(n, mesh, spec, p->tw, stats, dc_part, s);
   if (e) return e;
-  return finish_powspec(p, spec, thr, jstart, nb, window_p, flags, box, shot, keep_dc ? dc_part : nullptr, nullptr,
-                        partials, sums, counts, pk, nmodes, kmean, s);
+  return finish_powspec(p, j, s);
 }
 }  // namespace
 
@@ -271,15 +292,16 @@ int pk_scatter(const float* x, int64_t sx, const float* y, int64_t sy, const flo
                int64_t sw, int64_t np, float xmin, float ymin, float zmin, double box_size, int n, int stencil,
                int wrap_mode, const int* tile, const float* mesh_in, float* mesh_out, double* stats,
                void* workspace, size_t workspace_bytes, void* stream) {
-  ScatterArgs a{};
-  int e = make_args(&a, x, sx, y, sy, z, sz, w, sw, np, xmin, ymin, zmin, box_size, n, stencil, wrap_mode);
+  Job j{};
+  j.active = 1;
+  int e = make_args(&j.a, x, sx, y, sy, z, sz, w, sw, np, xmin, ymin, zmin, box_size, n, stencil, wrap_mode);
   if (e) return e;
   if (!mesh_out) return set_error(PK_E_INVALID, "null mesh_out");
-  TileGeom g{};
   bool direct = false;
-  e = make_tile_geom(n, np, stencil, tile, &g, &direct);
+  e = make_tile_geom(n, np, stencil, tile, &j.g, &direct);
   if (e) return e;
-  return scatter_run(a, g, direct, mesh_in, mesh_out, stats, workspace, workspace_bytes, as_stream(stream));
+  j.stats = stats;
+  return scatter_run(j, direct, mesh_in, mesh_out, workspace, workspace_bytes, as_stream(stream));
 }
 
 int pk_r2c(pk_plan* p, const float* mesh, const double* stats, void* spectrum, void* stream) {
@@ -289,7 +311,9 @@ int pk_r2c(pk_plan* p, const float* mesh, const double* stats, void* spectrum, v
   hipStream_t s = as_stream(stream);
   float2* spec = static_cast<float2*>(spectrum);
   e = fft_zpass(p->n, mesh, spec, p->tw, stats, nullptr, s);
-  if (!e) e = fft_ypass(p->n, spec, p->tw, s);
+  Job j{};
+  j.spec = spec;
+  if (!e) e = fft_ypass(p->n, j, p->tw, s);
   if (!e) e = fft_xpass(p->n, spec, p->tw, s);
   return e;
 }
@@ -392,27 +416,317 @@ int pk_estimate(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t 
   if (workspace_bytes < ws.bytes) return set_error(PK_E_WORKSPACE, "estimate workspace %zu < %zu bytes", workspace_bytes, ws.bytes);
   if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
   hipStream_t s = as_stream(stream);
-  double* st = stats ? stats : ws.stats;
+  Job j{};
+  j.active = 1;
+  j.a = a;
+  j.g = g;
+  j.stats = stats ? stats : ws.stats;
   if (!ws.fused) {
-    e = scatter_run(a, g, direct, nullptr, ws.mesh, st, ws.scatter_ws, ws.scatter_bytes, s);
+    e = scatter_run(j, direct, nullptr, ws.mesh, ws.scatter_ws, ws.scatter_bytes, s);
     if (e) return e;
-    return run_powspec(p, ws.mesh, st, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size, shot_noise,
-                       ws.spec, sums, counts, pk, nmodes, kmean, ws.powspec_ws, ws.powspec_bytes, s);
+    return run_powspec(p, ws.mesh, j.stats, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size,
+                       shot_noise, ws.spec, sums, counts, pk, nmodes, kmean, ws.powspec_ws, ws.powspec_bytes, s);
   }
   // fused: sort -> deposit + z pass -> mass -> y pass -> x pass + bin -> finalize scaled by (n^3 / M)^2
-  const void* sorted = nullptr;
-  const int* key_start = nullptr;
-  double* tile_mass = nullptr;
-  const double* aux = nullptr;
-  e = scatter_run(a, g, false, nullptr, nullptr, st, ws.scatter_ws, ws.scatter_bytes, s, &sorted, &key_start,
-                  &tile_mass, &aux);
+  e = scatter_run(j, false, nullptr, nullptr, ws.scatter_ws, ws.scatter_bytes, s);
   if (e) return e;
-  e = fused_deposit_zpass(p->n, a, g, key_start, sorted, p->tw, st, aux, ws.spec, tile_mass, s);
+  set_bin_fields(j, p, ws.spec, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size, shot_noise, sums,
+                 counts, pk, nmodes, kmean);
+  j.partials = static_cast<double*>(ws.powspec_ws);
+  j.scale_stats = j.stats;
+  e = fused_deposit_zpass(p->n, j, p->tw, s);
   if (e) return e;
-  e = mass_finalize(tile_mass, g.ntiles, st, p->n, s);
+  e = mass_finalize(j, s);
   if (e) return e;
-  return finish_powspec(p, ws.spec, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size, shot_noise,
-                        nullptr, st, static_cast<double*>(ws.powspec_ws), sums, counts, pk, nmodes, kmean, s);
+  return finish_powspec(p, j, s);
+}
+
+// ------------------------------------------------------------------------------------
+// batches
+// ------------------------------------------------------------------------------------
+}  // extern "C"
+
+struct pk_batch {
+  pk_plan* plan;
+  int n;
+  int max_jobs;
+  int64_t max_np;
+  int max_nb;
+  int flags;
+  TileGeom cap;          // capacity geometry: TSC classes, max_np particles
+  size_t slot_bytes;
+  char* slots;           // per-job scratch slots (caller workspace)
+  Job* d_jobs;           // device job table (head of the workspace)
+  Job* h_jobs[2];        // pinned host tables, used alternately
+  hipEvent_t uploaded[2];
+  bool pending[2];
+  int cur;
+  int njobs;
+  BatchDims dims;
+  int gmax;              // x-pass blocks of one job alone (partials rows per slot)
+  std::vector<Job> slot;  // scratch pointers of every slot
+  struct Graph {
+    BatchDims d;
+    hipGraphExec_t exec;
+    hipGraph_t graph;
+  };
+  std::vector<Graph> graphs;
+  std::vector<BatchDims> seen;   // shapes already run once without capture
+};
+
+namespace pk {
+namespace {
+thread_local bool t_capturing = false;
+}
+void timing_suspend(bool on) { t_capturing = on; }
+bool timing_suspended() { return t_capturing; }
+}  // namespace pk
+
+namespace {
+bool same_dims(const BatchDims& a, const BatchDims& b) {
+  return a.njobs == b.njobs && a.max_nblk == b.max_nblk && a.max_nkeys == b.max_nkeys && a.max_nb == b.max_nb;
+}
+
+int batch_geometry(int n, int64_t max_np, const int* tile, TileGeom* cap) {
+  bool direct = false;
+  int e = make_tile_geom(n, max_np, PK_STENCIL_TSC, tile, cap, &direct);
+  if (e) return e;
+  if (direct || !fused_supported(n, *cap))
+    return set_error(PK_E_UNSUPPORTED, "batches need the sorted fused path, which n=%d with %d rows per tile lacks", n,
+                     cap->ty);
+  return PK_OK;
+}
+
+struct SlotLayout {
+  size_t spec, partials, scatter, bytes;
+};
+
+SlotLayout slot_layout(int n, int64_t max_np, int max_nb, const TileGeom& cap, int gmax) {
+  SlotLayout L{};
+  L.spec = 0;
+  size_t off = align_up((size_t)(n / 2 + 1) * n * n * sizeof(float2));
+  L.partials = off;
+  off += align_up((size_t)gmax * (size_t)max_nb * 5 * sizeof(double));
+  L.scatter = off;
+  Job cj{};
+  cj.g = cap;
+  cj.a.np = max_np;
+  off += carve_scatter(cj, false, nullptr);
+  L.bytes = align_up(off);
+  return L;
+}
+
+size_t table_bytes(int max_jobs) { return align_up((size_t)max_jobs * sizeof(Job)); }
+
+int batch_launch(pk_batch* b, hipStream_t s) {
+  const int n = b->n, nj = b->njobs;
+  const float2* tw = b->plan->tw;
+  int e = scatter_sort_jobs(b->d_jobs, b->dims, s);
+  if (e) return e;
+  e = fused_deposit_zpass_jobs(n, b->cap.ty, b->d_jobs, nj, b->cap.ntiles, tw, s);
+  if (e) return e;
+  e = mass_finalize_jobs(b->d_jobs, nj, s);
+  if (e) return e;
+  const int gy = (ypass_grid(n) + nj - 1) / nj;
+  e = fft_ypass_jobs(n, b->d_jobs, nj, gy, tw, s);
+  if (e) return e;
+  int gx = (b->gmax + nj - 1) / nj;
+  if (gx < 1) gx = 1;
+  e = fft_xpass_bin_jobs(n, b->d_jobs, nj, b->dims.max_nb, gx, tw, s);
+  if (e) return e;
+  return bin_finalize_jobs(b->d_jobs, nj, b->dims.max_nb, gx, s);
+}
+}  // namespace
+
+extern "C" {
+
+int pk_batch_workspace_size(pk_plan* p, int max_jobs, int64_t max_np, int max_nb, const int* tile, size_t* bytes) {
+  int e = check_plan(p);
+  if (e) return e;
+  if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
+  if (max_jobs < 1 || max_jobs > 65535) return set_error(PK_E_INVALID, "max_jobs=%d out of range", max_jobs);
+  if (max_np < 0 || max_np > 0x7fffffffLL) return set_error(PK_E_INVALID, "max_np out of range");
+  if (max_nb < 1 || max_nb > (1 << 20)) return set_error(PK_E_INVALID, "max_nb=%d out of range", max_nb);
+  TileGeom cap{};
+  e = batch_geometry(p->n, max_np, tile, &cap);
+  if (e) return e;
+  const SlotLayout L = slot_layout(p->n, max_np, max_nb, cap, bin_grid(p->n, max_nb));
+  *bytes = table_bytes(max_jobs) + (size_t)max_jobs * L.bytes;
+  return PK_OK;
+}
+
+int pk_batch_create(pk_plan* p, int max_jobs, int64_t max_np, int max_nb, const int* tile, int flags, void* workspace,
+                    size_t workspace_bytes, pk_batch** out) {
+  if (!out) return set_error(PK_E_INVALID, "null batch output pointer");
+  *out = nullptr;
+  size_t need = 0;
+  int e = pk_batch_workspace_size(p, max_jobs, max_np, max_nb, tile, &need);
+  if (e) return e;
+  if (!workspace || workspace_bytes < need)
+    return set_error(PK_E_WORKSPACE, "batch workspace %zu < %zu bytes", workspace_bytes, need);
+  pk_batch* b = new pk_batch();
+  b->plan = p;
+  b->n = p->n;
+  b->max_jobs = max_jobs;
+  b->max_np = max_np;
+  b->max_nb = max_nb;
+  b->flags = flags;
+  batch_geometry(p->n, max_np, tile, &b->cap);
+  b->gmax = bin_grid(p->n, max_nb);
+  const SlotLayout L = slot_layout(p->n, max_np, max_nb, b->cap, b->gmax);
+  b->slot_bytes = L.bytes;
+  b->d_jobs = static_cast<Job*>(workspace);
+  b->slots = static_cast<char*>(workspace) + table_bytes(max_jobs);
+  b->slot.resize(max_jobs);
+  for (int i = 0; i < max_jobs; ++i) {
+    char* base = b->slots + (size_t)i * L.bytes;
+    Job& j = b->slot[i];
+    j = Job{};
+    j.g = b->cap;
+    j.a.np = max_np;
+    j.spec = reinterpret_cast<float2*>(base + L.spec);
+    j.partials = reinterpret_cast<double*>(base + L.partials);
+    carve_scatter(j, false, base + L.scatter);   // also points j.stats at the slot's own stats
+  }
+  for (int k = 0; k < 2; ++k) {
+    b->h_jobs[k] = nullptr;
+    b->uploaded[k] = nullptr;
+    b->pending[k] = false;
+  }
+  for (int k = 0; k < 2 && !e; ++k) {
+    e = check_hip(hipHostMalloc(reinterpret_cast<void**>(&b->h_jobs[k]), (size_t)max_jobs * sizeof(Job)),
+                  "pinned job table");
+    if (!e) e = check_hip(hipEventCreateWithFlags(&b->uploaded[k], hipEventDisableTiming), "job table event");
+  }
+  if (e) {
+    pk_batch_destroy(b);
+    return e == PK_E_LAUNCH ? PK_E_OOM : e;
+  }
+  b->cur = 0;
+  b->njobs = 0;
+  b->dims = BatchDims{0, 0, 0, 0};
+  *out = b;
+  return PK_OK;
+}
+
+int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) {
+  if (!b) return set_error(PK_E_INVALID, "null batch");
+  int e = check_plan(b->plan);
+  if (e) return e;
+  if (njobs < 0 || njobs > b->max_jobs) return set_error(PK_E_INVALID, "njobs=%d not in 0..%d", njobs, b->max_jobs);
+  if (njobs > 0 && !jobs) return set_error(PK_E_INVALID, "null job array");
+  const int k = b->cur ^ 1;   // the table not used by the latest upload
+  if (b->pending[k]) {
+    e = check_hip(hipEventSynchronize(b->uploaded[k]), "job table reuse");
+    if (e) return e;
+    b->pending[k] = false;
+  }
+  Job* h = b->h_jobs[k];
+  BatchDims d{njobs, 1, 1, 1};
+  const int tile[3] = {1, b->cap.ty, 0};
+  for (int i = 0; i < njobs; ++i) {
+    const pk_job& q = jobs[i];
+    Job j = b->slot[i];
+    e = make_args(&j.a, q.x, q.sx, q.y, q.sy, q.z, q.sz, q.w, q.sw, q.np, q.xmin, q.ymin, q.zmin, q.box_size, b->n,
+                  q.stencil, q.wrap_mode);
+    if (!e && q.np > b->max_np) e = set_error(PK_E_INVALID, "job %d: %lld particles > batch capacity %lld", i,
+                                              (long long)q.np, (long long)b->max_np);
+    if (!e) e = check_bins(q.thr, q.jstart, q.nb, q.pk);
+    if (!e && q.nb > b->max_nb) e = set_error(PK_E_INVALID, "job %d: %d bins > batch capacity %d", i, q.nb, b->max_nb);
+    if (!e && (q.window_p < 0 || q.window_p > 3)) e = set_error(PK_E_INVALID, "job %d: window p=%d", i, q.window_p);
+    bool direct = false;
+    if (!e) e = make_tile_geom(b->n, q.np, q.stencil, tile, &j.g, &direct);
+    if (!e && (direct || !fused_supported(b->n, j.g))) e = set_error(PK_E_UNSUPPORTED, "job %d: no fused path", i);
+    if (e) return e;
+    j.active = 1;
+    if (q.stats) j.stats = q.stats;
+    set_bin_fields(j, b->plan, j.spec, q.thr, q.jstart, q.nb, q.window_p, q.flags | PK_FLAG_ZERO_DC, q.box_size,
+                   q.shot_noise, q.sums, q.counts, q.pk, q.nmodes, q.kmean);
+    j.dc_part = nullptr;
+    j.Gz = 0;
+    j.scale_stats = j.stats;
+    h[i] = j;
+    if (j.g.nblk > d.max_nblk) d.max_nblk = j.g.nblk;
+    if (j.g.nkeys > d.max_nkeys) d.max_nkeys = j.g.nkeys;
+    if (q.nb > d.max_nb) d.max_nb = q.nb;
+  }
+  hipStream_t s = as_stream(stream);
+  if (njobs > 0) {
+    e = check_hip(hipMemcpyAsync(b->d_jobs, h, (size_t)njobs * sizeof(Job), hipMemcpyHostToDevice, s),
+                  "job table upload");
+    if (e) return e;
+    e = check_hip(hipEventRecord(b->uploaded[k], s), "job table event");
+    if (e) return e;
+    b->pending[k] = true;
+  }
+  b->cur = k;
+  b->njobs = njobs;
+  b->dims = d;
+  return PK_OK;
+}
+
+int pk_batch_run(pk_batch* b, void* stream) {
+  if (!b) return set_error(PK_E_INVALID, "null batch");
+  int e = check_plan(b->plan);
+  if (e) return e;
+  if (b->njobs == 0) return PK_OK;
+  hipStream_t s = as_stream(stream);
+  bool graph = (b->flags & PK_BATCH_GRAPH) != 0 && s != nullptr;
+  if (graph) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) graph = false;
+  }
+  if (!graph) return batch_launch(b, s);
+  for (auto& g : b->graphs)
+    if (same_dims(g.d, b->dims)) return check_hip(hipGraphLaunch(g.exec, s), "batch graph launch");
+  bool seen = false;
+  for (auto& d : b->seen) seen = seen || same_dims(d, b->dims);
+  if (!seen) {   // first run of this shape: plain launches (sets kernel attributes outside capture)
+    b->seen.push_back(b->dims);
+    return batch_launch(b, s);
+  }
+  pk_batch::Graph g{b->dims, nullptr, nullptr};
+  e = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "batch capture begin");
+  if (e) return e;
+  timing_suspend(true);
+  const int le = batch_launch(b, s);
+  timing_suspend(false);
+  e = check_hip(hipStreamEndCapture(s, &g.graph), "batch capture end");
+  if (le) {
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    return le;
+  }
+  if (e) return e;
+  e = check_hip(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "batch graph instantiate");
+  if (e) {
+    (void)hipGraphDestroy(g.graph);
+    return e;
+  }
+  b->graphs.push_back(g);
+  return check_hip(hipGraphLaunch(g.exec, s), "batch graph launch");
+}
+
+int pk_batch_tile(pk_batch* b, int* tile) {
+  if (!b || !tile) return set_error(PK_E_INVALID, "null batch / tile");
+  tile[0] = 1;
+  tile[1] = b->cap.ty;
+  tile[2] = b->n;
+  return PK_OK;
+}
+
+int pk_batch_destroy(pk_batch* b) {
+  if (!b) return PK_OK;
+  for (auto& g : b->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (b->pending[k] && b->uploaded[k]) (void)hipEventSynchronize(b->uploaded[k]);
+    if (b->uploaded[k]) (void)hipEventDestroy(b->uploaded[k]);
+    if (b->h_jobs[k]) (void)hipHostFree(b->h_jobs[k]);
+  }
+  delete b;
+  return PK_OK;
 }
 
 int pk_timing_enable(int on) {

@@ -90,8 +90,7 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
 // y pass: contiguous rows of N complex, in place
 // ------------------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(FFTCfg<N>::NTY)
-k_ypass(float2* __restrict__ spec, const float2* __restrict__ twg) {
+__device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const float2* __restrict__ twg, int bx, int nbx) {
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CY, NT = Cf::NTY, CS = N + 1;
   constexpr int NTILES = (N / 2 + 1) * N / C;
@@ -99,7 +98,7 @@ k_ypass(float2* __restrict__ spec, const float2* __restrict__ twg) {
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
-  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+  for (int tile = bx; tile < NTILES; tile += nbx) {
     float2* row = spec + (size_t)tile * C * N;
     for (int i = tid; i < C * N; i += NT) buf[(i / N) * CS + (i % N)] = row[i];
     __syncthreads();
@@ -107,6 +106,19 @@ k_ypass(float2* __restrict__ spec, const float2* __restrict__ twg) {
     for (int i = tid; i < C * N; i += NT) row[i] = buf[(i / N) * CS + (i % N)];
     __syncthreads();
   }
+}
+
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass(float2* __restrict__ spec, const float2* __restrict__ twg) {
+  ypass_body<N>(spec, twg, blockIdx.x, gridDim.x);
+}
+
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_jobs(const Job* __restrict__ jobs,
+                                                               const float2* __restrict__ twg) {
+  const Job& j = jobs[blockIdx.y];
+  if (!j.active) return;
+  ypass_body<N>(j.spec, twg, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -156,10 +168,13 @@ constexpr int NACC = 5;       // S0, S2, S4, count, sum sqrt(m)
 // partials[block][nb][NACC] once at the end.
 // ------------------------------------------------------------------------------------
 template <int N, bool LH>
-__global__ void __launch_bounds__(FFTCfg<N>::NTX)
-k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, const float* __restrict__ wtab,
-            const int* __restrict__ thr_g, const int* __restrict__ jst_g, int nb, int jmax, int flags,
-            double* __restrict__ partials) {
+__device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __restrict__ twg, int bx, int nbx) {
+  const float2* __restrict__ spec = j.spec;
+  const float* __restrict__ wtab = j.wtab;
+  const int* __restrict__ thr_g = j.thr;
+  const int* __restrict__ jst_g = j.jst;
+  const int nb = j.nb;
+  const int flags = j.flags;
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
   constexpr int YT = N / C;
@@ -175,7 +190,7 @@ k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, con
   const float* wt = wtab;
   const int* jst = jst_g;
   const int tid = threadIdx.x;
-  double* gpart = partials + (size_t)blockIdx.x * nb * NACC;
+  double* gpart = j.partials + (size_t)bx * nb * NACC;
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
     for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0.0;
@@ -187,7 +202,7 @@ k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, con
   __syncthreads();
   const int* thr = LH ? thr_s : thr_g;
   const bool want_k = (flags & PK_FLAG_KMEAN) != 0;
-  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+  for (int tile = bx; tile < NTILES; tile += nbx) {
     const int kz = tile / YT, yt = tile % YT;
     const float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
     for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
@@ -253,18 +268,38 @@ k_xpass_bin(const float2* __restrict__ spec, const float2* __restrict__ twg, con
   }
 }
 
+template <int N, bool LH>
+__global__ void __launch_bounds__(FFTCfg<N>::NTX) k_xpass_bin(Job j, const float2* __restrict__ twg) {
+  xpass_bin_body<N, LH>(j, twg, blockIdx.x, gridDim.x);
+}
+
+template <int N, bool LH>
+__global__ void __launch_bounds__(FFTCfg<N>::NTX) k_xpass_bin_jobs(const Job* __restrict__ jobs,
+                                                                   const float2* __restrict__ twg) {
+  const Job& j = jobs[blockIdx.y];
+  if (!j.active) return;
+  xpass_bin_body<N, LH>(j, twg, blockIdx.x, gridDim.x);
+}
+
 // ------------------------------------------------------------------------------------
 // reduce the per-block partials in a fixed order and normalise (src/correlations.py:49-54)
 // ------------------------------------------------------------------------------------
 constexpr int FIN_NT = 256;
 
-__global__ void __launch_bounds__(FIN_NT)
-k_bin_finalize(const double* __restrict__ partials, int G, int nb, const int* __restrict__ thr,
-               const double* __restrict__ dc_part, int Gz, const double* __restrict__ scale_stats, double norm,
-               double shot, double* __restrict__ sums, int64_t* __restrict__ counts, float* __restrict__ pk,
-               float* __restrict__ nmodes, double* __restrict__ kmean) {
+__device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   __shared__ double red[FIN_NT];
-  const int b = blockIdx.x;   // one block per bin
+  const double* __restrict__ partials = j.partials;
+  const int nb = j.nb;
+  const int* __restrict__ thr = j.thr;
+  const double* __restrict__ dc_part = j.dc_part;
+  const int Gz = j.Gz;
+  const double* __restrict__ scale_stats = j.scale_stats;
+  const double norm = j.norm, shot = j.shot;
+  double* __restrict__ sums = j.sums;
+  int64_t* __restrict__ counts = j.counts;
+  float* __restrict__ pk = j.pk;
+  float* __restrict__ nmodes = j.nmodes;
+  double* __restrict__ kmean = (j.flags & PK_FLAG_KMEAN) ? j.kmean : nullptr;
   double part[NACC] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int g = threadIdx.x; g < G; g += FIN_NT) {
     const double* p = partials + ((size_t)g * nb + b) * NACC;
@@ -305,6 +340,14 @@ k_bin_finalize(const double* __restrict__ partials, int G, int nb, const int* __
   if (kmean) kmean[b] = acc[4] / cnt;
 }
 
+__global__ void __launch_bounds__(FIN_NT) k_bin_finalize(Job j, int G) { bin_finalize_body(j, G, blockIdx.x); }
+
+__global__ void __launch_bounds__(FIN_NT) k_bin_finalize_jobs(const Job* __restrict__ jobs, int G) {
+  const Job& j = jobs[blockIdx.y];
+  if (!j.active || (int)blockIdx.x >= j.nb) return;
+  bin_finalize_body(j, G, blockIdx.x);
+}
+
 // ------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------
@@ -322,9 +365,17 @@ static int persistent_grid(K kernel, int nt, int ntiles) {
 
 template <int N> struct FFTLaunch {
   using Cf = FFTCfg<N>;
+  static constexpr int ZTILES = N * (N / Cf::CZ);
+  static constexpr int YTILES = (N / 2 + 1) * N / Cf::CY;
+  static constexpr int XTILES = (N / 2 + 1) * (N / Cf::CX);
   static int zgrid() {
     static int grid = 0;
-    if (!grid) grid = persistent_grid(k_zpass<N>, Cf::NTZ, N * (N / Cf::CZ));
+    if (!grid) grid = persistent_grid(k_zpass<N>, Cf::NTZ, ZTILES);
+    return grid;
+  }
+  static int ygrid() {
+    static int grid = 0;
+    if (!grid) grid = persistent_grid(k_ypass<N>, Cf::NTY, YTILES);
     return grid;
   }
   static int zpass(const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
@@ -335,39 +386,49 @@ template <int N> struct FFTLaunch {
     return check_hip(hipGetLastError(), "z pass launch");
   }
   static int ypass(float2* spec, const float2* tw, hipStream_t s) {
-    static int grid = 0;
-    if (!grid) grid = persistent_grid(k_ypass<N>, Cf::NTY, (N / 2 + 1) * N / Cf::CY);
+    const int grid = ygrid();
     StageTimer t(ST_YPASS, s);
     hipLaunchKernelGGL(k_ypass<N>, dim3(grid), dim3(Cf::NTY), 0, s, spec, tw);
     return check_hip(hipGetLastError(), "y pass launch");
   }
+  static int ypass_jobs(const Job* jobs, int njobs, int gy, const float2* tw, hipStream_t s) {
+    if (gy < 1) gy = 1;
+    if (gy > YTILES) gy = YTILES;
+    StageTimer t(ST_YPASS, s);
+    hipLaunchKernelGGL(k_ypass_jobs<N>, dim3(gy, njobs), dim3(Cf::NTY), 0, s, jobs, tw);
+    return check_hip(hipGetLastError(), "batched y pass launch");
+  }
   static int xpass(float2* spec, const float2* tw, hipStream_t s) {
     static int grid = 0;
-    if (!grid) grid = persistent_grid(k_xpass<N>, Cf::NTX, (N / 2 + 1) * (N / Cf::CX));
+    if (!grid) grid = persistent_grid(k_xpass<N>, Cf::NTX, XTILES);
     StageTimer t(ST_XPASS_PLAIN, s);
     hipLaunchKernelGGL(k_xpass<N>, dim3(grid), dim3(Cf::NTX), 0, s, spec, tw);
     return check_hip(hipGetLastError(), "x pass launch");
   }
   static int bin_grid(bool lh) {
     static int g_l = 0, g_g = 0;
-    const int ntiles = (N / 2 + 1) * (N / Cf::CX);
     if (lh) {
-      if (!g_l) g_l = persistent_grid(k_xpass_bin<N, true>, Cf::NTX, ntiles);
+      if (!g_l) g_l = persistent_grid(k_xpass_bin<N, true>, Cf::NTX, XTILES);
       return g_l;
     }
-    if (!g_g) g_g = persistent_grid(k_xpass_bin<N, false>, Cf::NTX, ntiles);
+    if (!g_g) g_g = persistent_grid(k_xpass_bin<N, false>, Cf::NTX, XTILES);
     return g_g;
   }
-  static int xpass_bin(const float2* spec, const float2* tw, const float* wtab, const int* thr, const int* jst,
-                       int nb, int jmax, int flags, double* partials, int grid, hipStream_t s) {
+  static int xpass_bin(const Job& j, const float2* tw, int grid, hipStream_t s) {
     StageTimer t(ST_XPASS_BIN, s);
-    if (nb <= HB_LDS)
-      hipLaunchKernelGGL((k_xpass_bin<N, true>), dim3(grid), dim3(Cf::NTX), 0, s, spec, tw, wtab, thr, jst, nb, jmax,
-                         flags, partials);
+    if (j.nb <= HB_LDS)
+      hipLaunchKernelGGL((k_xpass_bin<N, true>), dim3(grid), dim3(Cf::NTX), 0, s, j, tw);
     else
-      hipLaunchKernelGGL((k_xpass_bin<N, false>), dim3(grid), dim3(Cf::NTX), 0, s, spec, tw, wtab, thr, jst, nb,
-                         jmax, flags, partials);
+      hipLaunchKernelGGL((k_xpass_bin<N, false>), dim3(grid), dim3(Cf::NTX), 0, s, j, tw);
     return check_hip(hipGetLastError(), "x pass + bin launch");
+  }
+  static int xpass_bin_jobs(const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s) {
+    StageTimer t(ST_XPASS_BIN, s);
+    if (max_nb <= HB_LDS)
+      hipLaunchKernelGGL((k_xpass_bin_jobs<N, true>), dim3(gx, njobs), dim3(Cf::NTX), 0, s, jobs, tw);
+    else
+      hipLaunchKernelGGL((k_xpass_bin_jobs<N, false>), dim3(gx, njobs), dim3(Cf::NTX), 0, s, jobs, tw);
+    return check_hip(hipGetLastError(), "batched x pass + bin launch");
   }
 };
 
@@ -392,6 +453,15 @@ int zpass_grid(int n) {
   return 0;
 }
 
+int ypass_grid(int n) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::ygrid();
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return 0;
+}
+
 int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
               hipStream_t s) {
   switch (n) {
@@ -402,9 +472,18 @@ int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const do
   return set_error(PK_E_INVALID, "unsupported n=%d", n);
 }
 
-int fft_ypass(int n, float2* spec, const float2* tw, hipStream_t s) {
+int fft_ypass(int n, const Job& j, const float2* tw, hipStream_t s) {
   switch (n) {
-#define PK_CASE(V) case V: return FFTLaunch<V>::ypass(spec, tw, s);
+#define PK_CASE(V) case V: return FFTLaunch<V>::ypass(j.spec, tw, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int fft_ypass_jobs(int n, const Job* jobs, int njobs, int gy, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::ypass_jobs(jobs, njobs, gy, tw, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }
@@ -429,23 +508,34 @@ int bin_grid(int n, int nb) {
   return 0;
 }
 
-int fft_xpass_bin(int n, const float2* spec, const float2* tw, const float* wtab, const int* thr, const int* jst,
-                  int nb, int jmax, int flags, double* partials, int grid, hipStream_t s) {
+int fft_xpass_bin(int n, const Job& j, const float2* tw, int grid, hipStream_t s) {
   switch (n) {
-#define PK_CASE(V) case V: return FFTLaunch<V>::xpass_bin(spec, tw, wtab, thr, jst, nb, jmax, flags, partials, grid, s);
+#define PK_CASE(V) case V: return FFTLaunch<V>::xpass_bin(j, tw, grid, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }
   return set_error(PK_E_INVALID, "unsupported n=%d", n);
 }
 
-int bin_finalize(const double* partials, int G, int nb, const int* thr, const double* dc_part, int Gz,
-                 const double* scale_stats, double norm, double shot, double* sums, int64_t* counts, float* pk,
-                 float* nmodes, double* kmean, hipStream_t s) {
+int fft_xpass_bin_jobs(int n, const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::xpass_bin_jobs(jobs, njobs, max_nb, gx, tw, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int bin_finalize(const Job& j, int G, hipStream_t s) {
   StageTimer t(ST_FINALIZE, s);
-  hipLaunchKernelGGL(k_bin_finalize, dim3(nb), dim3(FIN_NT), 0, s, partials, G, nb, thr, dc_part, Gz, scale_stats,
-                     norm, shot, sums, counts, pk, nmodes, kmean);
+  hipLaunchKernelGGL(k_bin_finalize, dim3(j.nb), dim3(FIN_NT), 0, s, j, G);
   return check_hip(hipGetLastError(), "bin finalize launch");
+}
+
+int bin_finalize_jobs(const Job* jobs, int njobs, int max_nb, int G, hipStream_t s) {
+  StageTimer t(ST_FINALIZE, s);
+  hipLaunchKernelGGL(k_bin_finalize_jobs, dim3(max_nb, njobs), dim3(FIN_NT), 0, s, jobs, G);
+  return check_hip(hipGetLastError(), "batched bin finalize launch");
 }
 
 }  // namespace pk

@@ -16,10 +16,18 @@
 
 namespace pk {
 
-template <int N> struct FusedCfg {
+// Row-bundle height of the default tile (largest divisor of N that is <= 32, the same rule
+// as make_tile_geom) and of the second compiled variant (16 rows, N >= 64): the autotuner
+// picks between them.
+template <int N> struct FusedTY {
+  static constexpr int DEF = N <= 32 ? N : (N % 32 == 0 ? 32 : (N % 24 == 0 ? 24 : 16));
+  static constexpr int ALT = (N >= 64 && N % 16 == 0 && DEF != 16) ? 16 : DEF;
+};
+
+template <int N, int TY_> struct FusedCfg {
   static constexpr int NC = N / 2;
   static constexpr int CS = NC + 1;                      // complex row stride in LDS
-  static constexpr int TY = N <= 32 ? N : (N % 32 == 0 ? 32 : (N % 24 == 0 ? 24 : 16));
+  static constexpr int TY = TY_;
   static constexpr int NT = cmin(1024, cmax(64, (TY * NC / 8) / 64 * 64));
   // deposit into 64-bit fixed-point cells when they fit (the FFT rows alias them afterwards)
   static constexpr bool FIXED = (size_t)TY * N * 8 <= FIXED_LDS_MAX;
@@ -28,22 +36,21 @@ template <int N> struct FusedCfg {
   static constexpr int PER = (TY * N + NT - 1) / NT;     // cells per thread in the conversion
 };
 
-template <int N>
-__global__ void __launch_bounds__(FusedCfg<N>::NT)
-k_fused_deposit_zpass(ScatterArgs a, TileGeom g, const int* __restrict__ key_start, const void* __restrict__ sorted,
-                      const float2* __restrict__ twg, const double* __restrict__ stats, const double* __restrict__ aux,
-                      float2* __restrict__ spec, double* __restrict__ tile_mass) {
-  using Cf = FusedCfg<N>;
+template <int N, int TY>
+__device__ __forceinline__ void fused_body(const Job& j, int t, const float2* __restrict__ twg) {
+  using Cf = FusedCfg<N, TY>;
   constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
   extern __shared__ float2 buf[];   // FFT rows: C rows of CS complex = 2 * CS floats per row
   __shared__ float2 tw[N];
   __shared__ int s_key[MAX_TILE_BUCKETS], s_base[MAX_TILE_BUCKETS], s_pre[MAX_TILE_BUCKETS + 1];
   __shared__ int s_nbk;
   __shared__ double red[NT];
+  const ScatterArgs a = j.a;
+  const TileGeom g = j.g;
+  const void* sorted = j.sorted;
   float* rows = reinterpret_cast<float*>(buf);
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(buf);   // [C][N] when FIXED
   const int tid = threadIdx.x;
-  const int t = blockIdx.x;
   const int xt = t / g.nyt, yt = t % g.nyt;
   const int y0 = yt * C;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
@@ -52,34 +59,34 @@ k_fused_deposit_zpass(ScatterArgs a, TileGeom g, const int* __restrict__ key_sta
   } else {
     for (int i = tid; i < C * 2 * CS; i += NT) rows[i] = 0.0f;
   }
-  gather_buckets(g, N, a.stencil == PK_STENCIL_TSC, a.stencil == PK_STENCIL_CIC_REF, xt, yt, key_start, s_key,
+  gather_buckets(g, N, a.stencil == PK_STENCIL_TSC, a.stencil == PK_STENCIL_CIC_REF, xt, yt, j.key_start, s_key,
                  s_base, s_pre, &s_nbk);
-  const float scale = (float)aux[0];
+  const float scale = (float)j.aux[0];
   deposit_buckets(a, g, sorted, s_base, s_pre, s_nbk, NT, xt, y0, [&](int ly, int iz, float wt) {
     if constexpr (Cf::FIXED) atomicAdd(&acc[ly * N + iz], to_fixed(wt, scale));
     else atomicAdd(&rows[ly * 2 * CS + iz], wt);
   });
   __syncthreads();
   // tile mass (FP64) and rho - c into the FFT rows
-  const float c0 = (float)stats[3];
+  const float c0 = (float)j.stats[3];
   double msum = 0.0;
   if constexpr (Cf::FIXED) {
-    const double inv = aux[1];
+    const double inv = j.aux[1];
     float v[Cf::PER];
 #pragma unroll
-    for (int j = 0; j < Cf::PER; ++j) {
-      const int i = tid + j * NT;
+    for (int k = 0; k < Cf::PER; ++k) {
+      const int i = tid + k * NT;
       if (i < C * N) {
         const double q = (double)(long long)acc[i] * inv;
         msum += q;
-        v[j] = (float)q - c0;
+        v[k] = (float)q - c0;
       }
     }
     __syncthreads();   // every cell read before the float rows overwrite them
 #pragma unroll
-    for (int j = 0; j < Cf::PER; ++j) {
-      const int i = tid + j * NT;
-      if (i < C * N) rows[(i / N) * 2 * CS + (i % N)] = v[j];
+    for (int k = 0; k < Cf::PER; ++k) {
+      const int i = tid + k * NT;
+      if (i < C * N) rows[(i / N) * 2 * CS + (i % N)] = v[k];
     }
   } else {
     for (int i = tid; i < C * N; i += NT) {
@@ -91,9 +98,10 @@ k_fused_deposit_zpass(ScatterArgs a, TileGeom g, const int* __restrict__ key_sta
     }
   }
   const double m = block_sum_f64(msum, red, NT);   // ends with a barrier
-  if (tid == 0) tile_mass[t] = m;
+  if (tid == 0) j.tile_mass[t] = m;
   // z transform of the C rows (pairs of reals as NC complex, then the split)
   lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+  float2* spec = j.spec;
   for (int i = tid; i < (NC + 1) * C; i += NT) {
     const int c = i % C, k = i / C;
     const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
@@ -106,23 +114,52 @@ k_fused_deposit_zpass(ScatterArgs a, TileGeom g, const int* __restrict__ key_sta
   }
 }
 
-template <int N> struct FusedLaunch {
-  using Cf = FusedCfg<N>;
-  static int run(const ScatterArgs& a, const TileGeom& g, const int* key_start, const void* sorted,
-                 const float2* tw, const double* stats, const double* aux, float2* spec, double* tile_mass,
-                 hipStream_t s) {
-    static bool attr = false;
-    if (!attr && Cf::LDS > 65536) {
-      int e = check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fused_deposit_zpass<N>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)Cf::LDS),
-                        "LDS attribute");
+template <int N, int TY>
+__global__ void __launch_bounds__((FusedCfg<N, TY>::NT)) k_fused_deposit_zpass(Job j, const float2* __restrict__ tw) {
+  fused_body<N, TY>(j, blockIdx.x, tw);
+}
+
+template <int N, int TY>
+__global__ void __launch_bounds__((FusedCfg<N, TY>::NT))
+k_fused_deposit_zpass_jobs(const Job* __restrict__ jobs, const float2* __restrict__ tw) {
+  const Job& j = jobs[blockIdx.y];
+  if (!j.active || (int)blockIdx.x >= j.g.ntiles) return;
+  fused_body<N, TY>(j, blockIdx.x, tw);
+}
+
+template <int N, int TY> struct FusedLaunch {
+  using Cf = FusedCfg<N, TY>;
+  static bool ok(const TileGeom& g) {
+    return g.ty == TY && g.fixed == (int)Cf::FIXED && Cf::LDS + 16384 <= 160 * 1024;
+  }
+  template <typename K>
+  static int attr(K k) {
+    if (Cf::LDS <= 65536) return PK_OK;
+    return check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)Cf::LDS),
+                     "LDS attribute");
+  }
+  static int run(const Job& j, const float2* tw, hipStream_t s) {
+    static bool done = false;
+    if (!done) {
+      int e = attr(k_fused_deposit_zpass<N, TY>);
       if (e) return e;
+      done = true;
     }
-    attr = true;
     StageTimer t(ST_DEPOSIT, s);
-    hipLaunchKernelGGL(k_fused_deposit_zpass<N>, dim3(g.ntiles), dim3(Cf::NT), Cf::LDS, s, a, g, key_start, sorted,
-                       tw, stats, aux, spec, tile_mass);
+    hipLaunchKernelGGL((k_fused_deposit_zpass<N, TY>), dim3(j.g.ntiles), dim3(Cf::NT), Cf::LDS, s, j, tw);
     return check_hip(hipGetLastError(), "fused deposit + z pass launch");
+  }
+  static int run_jobs(const Job* jobs, int njobs, int ntiles, const float2* tw, hipStream_t s) {
+    static bool done = false;
+    if (!done) {
+      int e = attr(k_fused_deposit_zpass_jobs<N, TY>);
+      if (e) return e;
+      done = true;
+    }
+    StageTimer t(ST_DEPOSIT, s);
+    hipLaunchKernelGGL((k_fused_deposit_zpass_jobs<N, TY>), dim3(ntiles, njobs), dim3(Cf::NT), Cf::LDS, s, jobs, tw);
+    return check_hip(hipGetLastError(), "batched fused deposit + z pass launch");
   }
 };
 
@@ -131,23 +168,39 @@ template <int N> struct FusedLaunch {
 
 bool fused_supported(int n, const TileGeom& g) {
   switch (n) {
-#define PK_CASE(V) \
-  case V: return g.ty == FusedCfg<V>::TY && g.fixed == (int)FusedCfg<V>::FIXED && FusedCfg<V>::LDS + 16384 <= 160 * 1024;
+#define PK_CASE(V)                                                                    \
+  case V:                                                                             \
+    return FusedLaunch<V, FusedTY<V>::DEF>::ok(g) || FusedLaunch<V, FusedTY<V>::ALT>::ok(g);
     PK_FOR_EACH_FUSED_N(PK_CASE)
 #undef PK_CASE
     default: return false;
   }
 }
 
-int fused_deposit_zpass(int n, const ScatterArgs& a, const TileGeom& g, const int* key_start, const void* sorted,
-                        const float2* tw, const double* stats, const double* aux, float2* spec, double* tile_mass,
-                        hipStream_t s) {
+int fused_deposit_zpass(int n, const Job& j, const float2* tw, hipStream_t s) {
   switch (n) {
-#define PK_CASE(V) case V: return FusedLaunch<V>::run(a, g, key_start, sorted, tw, stats, aux, spec, tile_mass, s);
+#define PK_CASE(V)                                                                         \
+  case V:                                                                                  \
+    if (j.g.ty == FusedTY<V>::DEF) return FusedLaunch<V, FusedTY<V>::DEF>::run(j, tw, s);  \
+    if (j.g.ty == FusedTY<V>::ALT) return FusedLaunch<V, FusedTY<V>::ALT>::run(j, tw, s);  \
+    break;
     PK_FOR_EACH_FUSED_N(PK_CASE)
 #undef PK_CASE
   }
-  return set_error(PK_E_INVALID, "fused path: unsupported n=%d", n);
+  return set_error(PK_E_INVALID, "fused path: no kernel for n=%d, %d rows per tile", n, j.g.ty);
+}
+
+int fused_deposit_zpass_jobs(int n, int ty, const Job* jobs, int njobs, int ntiles, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V)                                                                                           \
+  case V:                                                                                                    \
+    if (ty == FusedTY<V>::DEF) return FusedLaunch<V, FusedTY<V>::DEF>::run_jobs(jobs, njobs, ntiles, tw, s);  \
+    if (ty == FusedTY<V>::ALT) return FusedLaunch<V, FusedTY<V>::ALT>::run_jobs(jobs, njobs, ntiles, tw, s);  \
+    break;
+    PK_FOR_EACH_FUSED_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "fused path: no kernel for n=%d, %d rows per tile", n, ty);
 }
 
 }  // namespace pk

@@ -8,20 +8,6 @@ namespace pk {
 
 int num_cus(int device);
 
-// ---- FFT passes (pk_fft.hip) --------------------------------------------------------
-bool fft_supported(int n);
-int zpass_grid(int n);
-int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
-              hipStream_t s);
-int fft_ypass(int n, float2* spec, const float2* tw, hipStream_t s);
-int fft_xpass(int n, float2* spec, const float2* tw, hipStream_t s);
-int bin_grid(int n, int nb);
-int fft_xpass_bin(int n, const float2* spec, const float2* tw, const float* wtab, const int* thr, const int* jst,
-                  int nb, int jmax, int flags, double* partials, int grid, hipStream_t s);
-int bin_finalize(const double* partials, int G, int nb, const int* thr, const double* dc_part, int Gz,
-                 const double* scale_stats, double norm, double shot, double* sums, int64_t* counts, float* pk,
-                 float* nmodes, double* kmean, hipStream_t s);
-
 // ---- mass assignment (pk_scatter.hip) -----------------------------------------------
 struct ScatterArgs {
   const float* x;
@@ -56,23 +42,89 @@ constexpr size_t FIXED_LDS_MAX = 131072;
 // are rint(weight * S) with S = 2^32 / 2^ceil(log2 max|w|), so |sum| < 2^63 for < 2^31 particles
 constexpr int AUX_LEN = 4;
 
+// One estimate's arguments and scratch as the shared kernels see them.  The plain entry
+// points pass a Job by value (kernel argument); batched launches read jobs[blockIdx.y]
+// from a device table, so one launch serves many catalogues of the same mesh side.
+struct Job {
+  ScatterArgs a;
+  TileGeom g;
+  int active;            // 0: every kernel skips this job (unused batch slot)
+  // sorted-deposit scratch
+  unsigned* mat;         // [nblk][nkeys] counts -> offsets inside each key
+  int* totals;           // [nkeys]
+  int* key_start;        // [nkeys + 1]
+  int* bsum;             // scan block sums
+  double* wsum;          // [nblk] weight sums
+  double* wmax;          // [nblk] max |w|
+  double* aux;           // [AUX_LEN] fixed-point scale
+  void* sorted;          // records
+  double* tile_mass;     // [ntiles]
+  double* stats;         // [PK_STATS_LEN]
+  // spectrum and binning
+  float2* spec;                // [n/2+1][n][n]
+  const float* wtab;           // window table of this job's exponent
+  const int* thr;              // [nb + 1]
+  const int* jst;              // isqrt start table
+  int nb;
+  int flags;
+  double* partials;            // [G][nb][NACC] x-pass block partials
+  const double* dc_part;       // FP64 k = 0 partials of the z pass (nullable)
+  int Gz;                      // number of dc_part entries
+  const double* scale_stats;   // sums scaled by scale_stats[1]^2 (fused contrast), nullable
+  double norm;                 // (box / n^2)^3
+  double shot;
+  double* sums;
+  int64_t* counts;
+  float* pk;
+  float* nmodes;
+  double* kmean;
+};
+
+// ---- FFT passes and binning (pk_fft.hip) -------------------------------------------------
+bool fft_supported(int n);
+int zpass_grid(int n);
+int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
+              hipStream_t s);
+int fft_xpass(int n, float2* spec, const float2* tw, hipStream_t s);
+int bin_grid(int n, int nb);   // x-pass blocks of one estimate
+// single job (by value) or a device table of njobs (then gy / gx = blocks per job)
+int fft_ypass(int n, const Job& j, const float2* tw, hipStream_t s);
+int fft_xpass_bin(int n, const Job& j, const float2* tw, int grid, hipStream_t s);
+int bin_finalize(const Job& j, int G, hipStream_t s);
+int ypass_grid(int n);
+int fft_ypass_jobs(int n, const Job* jobs, int njobs, int gy, const float2* tw, hipStream_t s);
+int fft_xpass_bin_jobs(int n, const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s);
+int bin_finalize_jobs(const Job* jobs, int njobs, int max_nb, int G, hipStream_t s);
+
+// ---- mass assignment (pk_scatter.hip) ------------------------------------------------------
 // validates/derives the tile geometry; tile == nullptr -> default rows per tile;
 // tile[0] == 0 -> direct global atomics; otherwise tile[1] = rows per tile
 int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g, bool* direct);
-// fused assignment + z pass (pk_fused.hip): deposit into LDS rows, subtract stats[3],
-// R2C along z, write the spectrum; per-tile FP64 masses into tile_mass
-int fused_deposit_zpass(int n, const ScatterArgs& a, const TileGeom& g, const int* key_start,
-                        const void* sorted, const float2* tw, const double* stats, const double* aux,
-                        float2* spec, double* tile_mass, hipStream_t s);
-bool fused_supported(int n, const TileGeom& g);
 size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np);
+// points the sorted-deposit scratch of j into base (nullptr: sizes only); returns bytes
+size_t carve_scatter(Job& j, bool direct, void* base);
 // mesh_out == nullptr (tiled mode only): stop after sorting; the caller runs the fused
-// deposit + z pass on the sorted records (sorted_out, key_start_out, tile_mass_out)
-int scatter_run(const ScatterArgs& a, const TileGeom& g, bool direct, const float* mesh_in, float* mesh_out,
-                double* stats, void* ws, size_t ws_bytes, hipStream_t s, const void** sorted_out = nullptr,
-                const int** key_start_out = nullptr, double** tile_mass_out = nullptr,
-                const double** aux_out = nullptr);
-int mass_finalize(const double* tile_mass, int ntiles, double* stats, int n, hipStream_t s);
+// deposit + z pass on j's sorted records
+int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void* ws, size_t ws_bytes,
+                hipStream_t s);
+int mass_finalize(const Job& j, hipStream_t s);
+// batched sort (count, scan, place) and mass finalize over a device job table; the
+// bounds are maxima over the jobs (blocks beyond a job's own extent return at once)
+struct BatchDims {
+  int njobs;
+  int max_nblk;
+  int max_nkeys;
+  int max_nb;
+};
+int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s);
+int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s);
+
+// ---- fused assignment + z pass (pk_fused.hip) ----------------------------------------------
+// deposit into LDS rows, subtract stats[3], R2C along z, write the spectrum; per-tile
+// FP64 masses into tile_mass.  Row-bundle heights with a compiled kernel: fused_ty_ok.
+bool fused_supported(int n, const TileGeom& g);
+int fused_deposit_zpass(int n, const Job& j, const float2* tw, hipStream_t s);
+int fused_deposit_zpass_jobs(int n, int ty, const Job* jobs, int njobs, int ntiles, const float2* tw, hipStream_t s);
 
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 

@@ -16,6 +16,10 @@
 //              once with coalesced stores: no global atomics and no zero fill.
 //   5. atomic: the (normally empty) irregular bucket is added with global atomics.
 // A direct mode (tile[0] == 0) adds every particle with global atomics instead.
+#include <mutex>
+#include <utility>
+#include <vector>
+
 #include "pk_particles.h"
 
 namespace pk {
@@ -34,21 +38,27 @@ __global__ void k_copy_or_zero(const float* src, float* dst, int64_t n) {  // sr
 
 constexpr int CNT_NT = 1024;
 
+// Batched wrappers: blockIdx.y selects the job; blocks beyond the job's own extent return
+// at once (the whole block, so no barrier is left waiting).
+#define PK_JOB_OR_RETURN(jobs, limit_expr)                              \
+  const Job& j = (jobs)[blockIdx.y];                                    \
+  if (!j.active || (int)blockIdx.x >= (limit_expr)) return;
+
 // ------------------------------------------------------------------------------------
 // 1. count: LDS histogram of one chunk -> matrix row; FP64 weight sum per block
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(CNT_NT) k_bucket_count(ScatterArgs a, TileGeom g, unsigned* __restrict__ mat,
-                                                         double* __restrict__ wsum_part,
-                                                         double* __restrict__ wmax_part) {
+__device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
   extern __shared__ unsigned hist2[];   // key k -> 16-bit half (k & 1) of word k >> 1
   __shared__ double red[CNT_NT];
   __shared__ unsigned wmax_bits;
+  const ScatterArgs a = j.a;
+  const TileGeom g = j.g;
   const int tid = threadIdx.x;
   const int words = (g.nkeys + 1) >> 1;
   for (int i = tid; i < words; i += CNT_NT) hist2[i] = 0u;
   if (tid == 0) wmax_bits = 0u;
   __syncthreads();
-  const int64_t lo = (int64_t)blockIdx.x * g.chunk;
+  const int64_t lo = (int64_t)bx * g.chunk;
   const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
   double ws = 0.0;
   float wm = 0.0f;
@@ -64,18 +74,27 @@ __global__ void __launch_bounds__(CNT_NT) k_bucket_count(ScatterArgs a, TileGeom
   }
   atomicMax(&wmax_bits, __float_as_uint(wm));   // non-negative floats order like their bits
   __syncthreads();
-  if (tid == 0) wmax_part[blockIdx.x] = (double)__uint_as_float(wmax_bits);
-  unsigned* row = mat + (size_t)blockIdx.x * g.nkeys;
+  if (tid == 0) j.wmax[bx] = (double)__uint_as_float(wmax_bits);
+  unsigned* row = j.mat + (size_t)bx * g.nkeys;
   for (int k = tid; k < g.nkeys; k += CNT_NT) row[k] = (hist2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
   const double t = block_sum_f64(ws, red, CNT_NT);
-  if (tid == 0) wsum_part[blockIdx.x] = t;
+  if (tid == 0) j.wsum[bx] = t;
+}
+__global__ void __launch_bounds__(CNT_NT) k_bucket_count(Job j) { bucket_count_body(j, blockIdx.x); }
+__global__ void __launch_bounds__(CNT_NT) k_bucket_count_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, j.g.nblk);
+  bucket_count_body(j, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
 // 2. column scan (offset of each block inside each key) + exclusive scan of key totals
 // ------------------------------------------------------------------------------------
-__global__ void k_col_scan(unsigned* __restrict__ mat, int nblk, int nkeys, int* __restrict__ totals) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int COL_NT = 256;
+
+__device__ __forceinline__ void col_scan_body(const Job& j, int bx) {
+  const int nkeys = j.g.nkeys, nblk = j.g.nblk;
+  unsigned* mat = j.mat;
+  const int k = bx * COL_NT + threadIdx.x;
   if (k >= nkeys) return;
   unsigned run = 0;
   for (int b = 0; b < nblk; ++b) {
@@ -84,7 +103,12 @@ __global__ void k_col_scan(unsigned* __restrict__ mat, int nblk, int nkeys, int*
     *p = run;
     run += c;
   }
-  totals[k] = (int)run;
+  j.totals[k] = (int)run;
+}
+__global__ void __launch_bounds__(COL_NT) k_col_scan(Job j) { col_scan_body(j, blockIdx.x); }
+__global__ void __launch_bounds__(COL_NT) k_col_scan_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, (j.g.nkeys + COL_NT - 1) / COL_NT);
+  col_scan_body(j, blockIdx.x);
 }
 
 constexpr int SCAN_NT = 1024;
@@ -107,10 +131,13 @@ __device__ int block_exclusive_scan(int v, int* sh, int& total) {
   return incl - v;
 }
 
-__global__ void __launch_bounds__(SCAN_NT) k_scan_local(const int* __restrict__ in, int* __restrict__ out,
-                                                        int* __restrict__ bsum, int n) {
+// key totals -> key starts: local scans of SCAN_BLOCK keys, a scan of the block sums, add
+__device__ __forceinline__ void scan_local_body(const Job& j, int bx) {
   __shared__ int sh[SCAN_NT];
-  const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER;
+  const int n = j.g.nkeys;
+  const int* in = j.totals;
+  int* out = j.key_start;
+  const int base = bx * SCAN_BLOCK + threadIdx.x * SCAN_PER;
   int v[SCAN_PER];
   int s = 0;
 #pragma unroll
@@ -125,60 +152,77 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_local(const int* __restrict__ 
     if (base + k < n) out[base + k] = run;
     run += v[k];
   }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+  if (threadIdx.x == 0) j.bsum[bx] = total;
+}
+__global__ void __launch_bounds__(SCAN_NT) k_scan_local(Job j) { scan_local_body(j, blockIdx.x); }
+__global__ void __launch_bounds__(SCAN_NT) k_scan_local_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, (j.g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  scan_local_body(j, blockIdx.x);
 }
 
-// scans the block sums; also sums the weight partials into stats[3] = sum(w) / n^3
-__global__ void __launch_bounds__(SCAN_NT) k_scan_top(int* __restrict__ bsum, int nblk, int* __restrict__ total_out,
-                                                      const double* __restrict__ wsum_part,
-                                                      const double* __restrict__ wmax_part, int nw,
-                                                      double* __restrict__ stats, double* __restrict__ aux,
-                                                      double ncells) {
+// scans the block sums; sums the weight partials into stats[3] = sum(w) / n^3 and sets
+// the fixed-point scale aux[0] = S, aux[1] = 1 / S from max |w|
+__device__ __forceinline__ void scan_top_body(const Job& j) {
   __shared__ int sh[SCAN_NT];
   __shared__ double shd[SCAN_NT];
   __shared__ unsigned long long wmax_bits;
+  const int nb2 = (j.g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  int* bsum = j.bsum;
   const int base = threadIdx.x * SCAN_PER;
   int v[SCAN_PER];
   int s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_PER; ++k) {
-    v[k] = (base + k < nblk) ? bsum[base + k] : 0;
+    v[k] = (base + k < nb2) ? bsum[base + k] : 0;
     s += v[k];
   }
   int total;
   int run = block_exclusive_scan(s, sh, total);
 #pragma unroll
   for (int k = 0; k < SCAN_PER; ++k) {
-    if (base + k < nblk) bsum[base + k] = run;
+    if (base + k < nb2) bsum[base + k] = run;
     run += v[k];
   }
-  if (threadIdx.x == 0) *total_out = total;
+  if (threadIdx.x == 0) j.key_start[j.g.nkeys] = total;
   double ws = 0.0, wm = 0.0;
   if (threadIdx.x == 0) wmax_bits = 0ull;
-  for (int i = threadIdx.x; i < nw; i += SCAN_NT) {
-    ws += wsum_part[i];
-    wm = fmax(wm, wmax_part[i]);
+  for (int i = threadIdx.x; i < j.g.nblk; i += SCAN_NT) {
+    ws += j.wsum[i];
+    wm = fmax(wm, j.wmax[i]);
   }
   const double t = block_sum_f64(ws, shd, SCAN_NT);   // ends with a barrier
   atomicMax(&wmax_bits, (unsigned long long)__double_as_longlong(wm));
   __syncthreads();
   if (threadIdx.x == 0) {
-    stats[3] = t / ncells;
+    const double n = (double)j.a.n;
+    j.stats[3] = t / (n * n * n);
     // fixed-point scale S = 2^32 / 2^ceil(log2 max|w|): every deposit |rint(w S)| <= 2^32
     int e = 0;
     const double wmax = __longlong_as_double((long long)wmax_bits);
     if (wmax > 0.0 && wmax < 1e300) (void)frexp(wmax, &e);
-    aux[0] = ldexp(1.0, 32 - e);
-    aux[1] = ldexp(1.0, e - 32);
+    j.aux[0] = ldexp(1.0, 32 - e);
+    j.aux[1] = ldexp(1.0, e - 32);
   }
 }
+__global__ void __launch_bounds__(SCAN_NT) k_scan_top(Job j) { scan_top_body(j); }
+__global__ void __launch_bounds__(SCAN_NT) k_scan_top_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, 1);
+  scan_top_body(j);
+}
 
-__global__ void __launch_bounds__(SCAN_NT) k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) {
-  const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER;
-  const int add = bsum[blockIdx.x];
+__device__ __forceinline__ void scan_add_body(const Job& j, int bx) {
+  const int n = j.g.nkeys;
+  int* out = j.key_start;
+  const int base = bx * SCAN_BLOCK + threadIdx.x * SCAN_PER;
+  const int add = j.bsum[bx];
 #pragma unroll
   for (int k = 0; k < SCAN_PER; ++k)
     if (base + k < n) out[base + k] += add;
+}
+__global__ void __launch_bounds__(SCAN_NT) k_scan_add(Job j) { scan_add_body(j, blockIdx.x); }
+__global__ void __launch_bounds__(SCAN_NT) k_scan_add_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, (j.g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  scan_add_body(j, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -186,15 +230,16 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_add(int* __restrict__ out, con
 // ------------------------------------------------------------------------------------
 constexpr int PLACE_UNROLL = 4;
 
-__global__ void __launch_bounds__(CNT_NT) k_bucket_place(ScatterArgs a, TileGeom g, const unsigned* __restrict__ mat,
-                                                         const int* __restrict__ key_start,
-                                                         void* __restrict__ sorted) {
+__device__ __forceinline__ void bucket_place_body(const Job& j, int bx) {
   extern __shared__ unsigned dest[];   // next write position of every key for this block
+  const ScatterArgs a = j.a;
+  const TileGeom g = j.g;
+  void* sorted = j.sorted;
   const int tid = threadIdx.x;
-  const unsigned* row = mat + (size_t)blockIdx.x * g.nkeys;
-  for (int k = tid; k < g.nkeys; k += CNT_NT) dest[k] = (unsigned)key_start[k] + row[k];
+  const unsigned* row = j.mat + (size_t)bx * g.nkeys;
+  for (int k = tid; k < g.nkeys; k += CNT_NT) dest[k] = (unsigned)j.key_start[k] + row[k];
   __syncthreads();
-  const int64_t lo = (int64_t)blockIdx.x * g.chunk;
+  const int64_t lo = (int64_t)bx * g.chunk;
   const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
   const bool has_w = a.w != nullptr;
   // PLACE_UNROLL independent particles per thread and iteration keep several loads in flight
@@ -213,6 +258,17 @@ __global__ void __launch_bounds__(CNT_NT) k_bucket_place(ScatterArgs a, TileGeom
       if (key >= 0) store_record(sorted, atomicAdd(&dest[key], 1u), has_w, px[u], py[u], pz[u], w[u]);
     }
   }
+}
+__global__ void __launch_bounds__(CNT_NT) k_bucket_place(Job j) { bucket_place_body(j, blockIdx.x); }
+__global__ void __launch_bounds__(CNT_NT) k_bucket_place_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, j.g.nblk);
+  bucket_place_body(j, blockIdx.x);
+}
+
+// stats[0..3] = 0 for every job of a batch (the plain path uses k_fill_f64)
+__global__ void k_zero_stats_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, 1);
+  if (threadIdx.x < PK_STATS_LEN) j.stats[threadIdx.x] = 0.0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -297,16 +353,21 @@ __global__ void __launch_bounds__(ATM_NT) k_atomic_deposit(ScatterArgs a, const 
 // ------------------------------------------------------------------------------------
 // mass finalize: stats[0] = M (fixed-order sum of tile masses + atomic part), stats[1] = n^3 / M
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(ATM_NT) k_mass_finalize(const double* __restrict__ tile_mass, int ntiles,
-                                                          double* __restrict__ stats, double ncells) {
+__device__ __forceinline__ void mass_finalize_body(const Job& j, int ntiles) {
   __shared__ double red[ATM_NT];
   double s = 0.0;
-  for (int i = threadIdx.x; i < ntiles; i += ATM_NT) s += tile_mass[i];
-  const double m = block_sum_f64(s, red, ATM_NT) + stats[2];
+  for (int i = threadIdx.x; i < ntiles; i += ATM_NT) s += j.tile_mass[i];
+  const double m = block_sum_f64(s, red, ATM_NT) + j.stats[2];
   if (threadIdx.x == 0) {
-    stats[0] = m;
-    stats[1] = ncells / m;
+    const double n = (double)j.a.n;
+    j.stats[0] = m;
+    j.stats[1] = n * n * n / m;
   }
+}
+__global__ void __launch_bounds__(ATM_NT) k_mass_finalize(Job j, int ntiles) { mass_finalize_body(j, ntiles); }
+__global__ void __launch_bounds__(ATM_NT) k_mass_finalize_jobs(const Job* __restrict__ jobs) {
+  PK_JOB_OR_RETURN(jobs, 1);
+  mass_finalize_body(j, j.g.ntiles);
 }
 
 // ------------------------------------------------------------------------------------
@@ -363,45 +424,6 @@ int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g,
 }
 
 namespace {
-struct ScatterWS {
-  double* stats;
-  unsigned* mat;
-  int* totals;
-  int* key_start;
-  int* bsum;
-  double* wsum;
-  double* wmax;
-  double* aux;
-  void* sorted;
-  double* tile_mass;
-  size_t bytes;
-};
-
-ScatterWS carve(const TileGeom& g, bool direct, int64_t np, void* base) {
-  ScatterWS w{};
-  size_t off = 0;
-  char* b = static_cast<char*>(base);
-  auto take = [&](size_t bytes) -> void* {
-    void* p = b ? b + off : nullptr;
-    off += align_up(bytes);
-    return p;
-  };
-  w.stats = static_cast<double*>(take(PK_STATS_LEN * sizeof(double)));
-  if (!direct) {
-    w.mat = static_cast<unsigned*>(take((size_t)g.nblk * g.nkeys * sizeof(unsigned)));
-    w.totals = static_cast<int*>(take((size_t)g.nkeys * sizeof(int)));
-    w.key_start = static_cast<int*>(take(((size_t)g.nkeys + 1) * sizeof(int)));
-    w.bsum = static_cast<int*>(take((size_t)SCAN_BLOCK * sizeof(int)));
-    w.wsum = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
-    w.wmax = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
-    w.aux = static_cast<double*>(take(AUX_LEN * sizeof(double)));
-    w.sorted = take((size_t)(np > 0 ? np : 1) * sizeof(float4));
-    w.tile_mass = static_cast<double*>(take((size_t)g.ntiles * sizeof(double)));
-  }
-  w.bytes = off;
-  return w;
-}
-
 int grid_for(int64_t items, int nt, int cap) {
   int64_t gsz = (items + nt - 1) / nt;
   if (gsz < 1) gsz = 1;
@@ -409,36 +431,73 @@ int grid_for(int64_t items, int nt, int cap) {
   return (int)gsz;
 }
 
+// raises a kernel's dynamic-LDS limit once per size (not again inside a graph capture)
 template <typename K>
 int allow_lds(K kernel, size_t bytes) {
   if (bytes <= 65536) return PK_OK;
-  return check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
-                   "LDS attribute");
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, size_t>> done;
+  const void* k = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& d : done)
+    if (d.first == k && d.second >= bytes) return PK_OK;
+  int e = check_hip(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes), "LDS attribute");
+  if (!e) done.emplace_back(k, bytes);
+  return e;
 }
+
+size_t hist_lds(int nkeys) { return (size_t)((nkeys + 1) / 2) * 4; }
+size_t dest_lds(int nkeys) { return (size_t)nkeys * 4; }
 }  // namespace
 
-size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np) {
-  return carve(g, direct, np, nullptr).bytes;
+size_t carve_scatter(Job& j, bool direct, void* base) {
+  size_t off = 0;
+  char* b = static_cast<char*>(base);
+  auto take = [&](size_t bytes) -> void* {
+    void* p = b ? b + off : nullptr;
+    off += align_up(bytes);
+    return p;
+  };
+  const TileGeom& g = j.g;
+  double* st = static_cast<double*>(take(PK_STATS_LEN * sizeof(double)));
+  if (!j.stats) j.stats = st;
+  if (!direct) {
+    j.mat = static_cast<unsigned*>(take((size_t)g.nblk * g.nkeys * sizeof(unsigned)));
+    j.totals = static_cast<int*>(take((size_t)g.nkeys * sizeof(int)));
+    j.key_start = static_cast<int*>(take(((size_t)g.nkeys + 1) * sizeof(int)));
+    j.bsum = static_cast<int*>(take((size_t)SCAN_BLOCK * sizeof(int)));
+    j.wsum = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
+    j.wmax = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
+    j.aux = static_cast<double*>(take(AUX_LEN * sizeof(double)));
+    j.sorted = take((size_t)(j.a.np > 0 ? j.a.np : 1) * sizeof(float4));
+    j.tile_mass = static_cast<double*>(take((size_t)g.ntiles * sizeof(double)));
+  }
+  return off;
 }
 
-int mass_finalize(const double* tile_mass, int ntiles, double* stats, int n, hipStream_t s) {
+size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np) {
+  Job j{};
+  j.g = g;
+  j.a.np = np;
+  return carve_scatter(j, direct, nullptr);
+}
+
+int mass_finalize(const Job& j, hipStream_t s) {
   StageTimer t(ST_MASS, s);
-  hipLaunchKernelGGL(k_mass_finalize, dim3(1), dim3(ATM_NT), 0, s, tile_mass, ntiles, stats,
-                     (double)n * (double)n * (double)n);
+  hipLaunchKernelGGL(k_mass_finalize, dim3(1), dim3(ATM_NT), 0, s, j, j.tile_mass ? j.g.ntiles : 0);
   return check_hip(hipGetLastError(), "mass finalize launch");
 }
 
-int scatter_run(const ScatterArgs& a, const TileGeom& g, bool direct, const float* mesh_in, float* mesh_out,
-                double* stats, void* ws, size_t ws_bytes, hipStream_t s, const void** sorted_out,
-                const int** key_start_out, double** tile_mass_out, const double** aux_out) {
-  ScatterWS w = carve(g, direct, a.np, ws);
-  if (ws_bytes < w.bytes) return set_error(PK_E_WORKSPACE, "scatter workspace %zu < %zu bytes", ws_bytes, w.bytes);
-  double* st = stats ? stats : w.stats;
+int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void* ws, size_t ws_bytes,
+                hipStream_t s) {
+  const size_t need = carve_scatter(j, direct, ws);
+  if (ws_bytes < need) return set_error(PK_E_WORKSPACE, "scatter workspace %zu < %zu bytes", ws_bytes, need);
+  const ScatterArgs& a = j.a;
+  const TileGeom& g = j.g;
   const int64_t ncell = (int64_t)a.n * a.n * a.n;
   {
     StageTimer t(ST_FILL, s);
-    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(PK_STATS_LEN), 0, s, st, PK_STATS_LEN, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(PK_STATS_LEN), 0, s, j.stats, PK_STATS_LEN, 0.0);
     if (direct) hipLaunchKernelGGL(k_copy_or_zero, dim3(grid_for(ncell, 256, 16384)), dim3(256), 0, s, mesh_in,
                                    mesh_out, ncell);
     int e = check_hip(hipGetLastError(), "scatter fill launch");
@@ -449,49 +508,42 @@ int scatter_run(const ScatterArgs& a, const TileGeom& g, bool direct, const floa
     if (a.np > 0) {
       StageTimer t(ST_ATOMIC, s);
       hipLaunchKernelGGL(k_atomic_deposit, dim3(grid_for(a.np, ATM_NT, 65535)), dim3(ATM_NT), 0, s, a,
-                         (const void*)nullptr, (const int*)nullptr, mesh_out, st);
+                         (const void*)nullptr, (const int*)nullptr, mesh_out, j.stats);
       int e = check_hip(hipGetLastError(), "atomic deposit launch");
       if (e) return e;
     }
-    return mass_finalize(nullptr, 0, st, a.n, s);
+    j.tile_mass = nullptr;
+    return mass_finalize(j, s);
   }
-  const size_t hist_bytes = (size_t)((g.nkeys + 1) / 2) * 4;
   {
     StageTimer t(ST_SORT_COUNT, s);
-    int e = allow_lds(k_bucket_count, hist_bytes);
+    const size_t lds = hist_lds(g.nkeys);
+    int e = allow_lds(k_bucket_count, lds);
     if (e) return e;
-    hipLaunchKernelGGL(k_bucket_count, dim3(g.nblk), dim3(CNT_NT), hist_bytes, s, a, g, w.mat, w.wsum, w.wmax);
+    hipLaunchKernelGGL(k_bucket_count, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
     e = check_hip(hipGetLastError(), "bucket count launch");
     if (e) return e;
   }
   {
     StageTimer t(ST_SCAN, s);
-    hipLaunchKernelGGL(k_col_scan, dim3((g.nkeys + 255) / 256), dim3(256), 0, s, w.mat, g.nblk, g.nkeys, w.totals);
+    hipLaunchKernelGGL(k_col_scan, dim3((g.nkeys + COL_NT - 1) / COL_NT), dim3(COL_NT), 0, s, j);
     const int nb2 = (g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    hipLaunchKernelGGL(k_scan_local, dim3(nb2), dim3(SCAN_NT), 0, s, w.totals, w.key_start, w.bsum, g.nkeys);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_NT), 0, s, w.bsum, nb2, w.key_start + g.nkeys,
-                       (const double*)w.wsum, (const double*)w.wmax, g.nblk, st, w.aux, (double)ncell);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb2), dim3(SCAN_NT), 0, s, w.key_start, (const int*)w.bsum, g.nkeys);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb2), dim3(SCAN_NT), 0, s, j);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_NT), 0, s, j);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb2), dim3(SCAN_NT), 0, s, j);
     int e = check_hip(hipGetLastError(), "scan launch");
     if (e) return e;
   }
   {
     StageTimer t(ST_SORT_PLACE, s);
-    const size_t dest_bytes = (size_t)g.nkeys * 4;
-    int e = allow_lds(k_bucket_place, dest_bytes);
+    const size_t lds = dest_lds(g.nkeys);
+    int e = allow_lds(k_bucket_place, lds);
     if (e) return e;
-    hipLaunchKernelGGL(k_bucket_place, dim3(g.nblk), dim3(CNT_NT), dest_bytes, s, a, g, (const unsigned*)w.mat,
-                       (const int*)w.key_start, w.sorted);
+    hipLaunchKernelGGL(k_bucket_place, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
     e = check_hip(hipGetLastError(), "bucket place launch");
     if (e) return e;
   }
-  if (!mesh_out) {  // the caller deposits (fused with the z pass)
-    if (sorted_out) *sorted_out = w.sorted;
-    if (key_start_out) *key_start_out = w.key_start;
-    if (tile_mass_out) *tile_mass_out = w.tile_mass;
-    if (aux_out) *aux_out = w.aux;
-    return PK_OK;
-  }
+  if (!mesh_out) return PK_OK;   // the caller deposits (fused with the z pass)
   {
     StageTimer t(ST_DEPOSIT, s);
     const size_t lds = (size_t)g.ty * a.n * (g.fixed ? 8 : 4);
@@ -499,25 +551,66 @@ int scatter_run(const ScatterArgs& a, const TileGeom& g, bool direct, const floa
     if (g.fixed) {
       e = allow_lds(k_rows_deposit<true>, lds);
       if (e) return e;
-      hipLaunchKernelGGL(k_rows_deposit<true>, dim3(g.ntiles), dim3(DEP_NT), lds, s, a, g, (const int*)w.key_start,
-                         (const void*)w.sorted, (const double*)w.aux, mesh_in, mesh_out, w.tile_mass);
+      hipLaunchKernelGGL(k_rows_deposit<true>, dim3(g.ntiles), dim3(DEP_NT), lds, s, a, g, (const int*)j.key_start,
+                         (const void*)j.sorted, (const double*)j.aux, mesh_in, mesh_out, j.tile_mass);
     } else {
       e = allow_lds(k_rows_deposit<false>, lds);
       if (e) return e;
-      hipLaunchKernelGGL(k_rows_deposit<false>, dim3(g.ntiles), dim3(DEP_NT), lds, s, a, g, (const int*)w.key_start,
-                         (const void*)w.sorted, (const double*)w.aux, mesh_in, mesh_out, w.tile_mass);
+      hipLaunchKernelGGL(k_rows_deposit<false>, dim3(g.ntiles), dim3(DEP_NT), lds, s, a, g, (const int*)j.key_start,
+                         (const void*)j.sorted, (const double*)j.aux, mesh_in, mesh_out, j.tile_mass);
     }
     e = check_hip(hipGetLastError(), "row deposit launch");
     if (e) return e;
   }
   if (a.stencil == PK_STENCIL_CIC_REF && a.np > 0) {
     StageTimer t(ST_ATOMIC, s);
-    hipLaunchKernelGGL(k_atomic_deposit, dim3(64), dim3(ATM_NT), 0, s, a, (const void*)w.sorted,
-                       (const int*)(w.key_start + g.nkeys - 1), mesh_out, st);
+    hipLaunchKernelGGL(k_atomic_deposit, dim3(64), dim3(ATM_NT), 0, s, a, (const void*)j.sorted,
+                       (const int*)(j.key_start + g.nkeys - 1), mesh_out, j.stats);
     int e = check_hip(hipGetLastError(), "irregular deposit launch");
     if (e) return e;
   }
-  return mass_finalize(w.tile_mass, g.ntiles, st, a.n, s);
+  return mass_finalize(j, s);
+}
+
+int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
+  const int nj = d.njobs;
+  {
+    StageTimer t(ST_FILL, s);
+    hipLaunchKernelGGL(k_zero_stats_jobs, dim3(1, nj), dim3(64), 0, s, jobs);
+    int e = check_hip(hipGetLastError(), "batch stats launch");
+    if (e) return e;
+  }
+  {
+    StageTimer t(ST_SORT_COUNT, s);
+    const size_t lds = hist_lds(d.max_nkeys);
+    int e = allow_lds(k_bucket_count_jobs, lds);
+    if (e) return e;
+    hipLaunchKernelGGL(k_bucket_count_jobs, dim3(d.max_nblk, nj), dim3(CNT_NT), lds, s, jobs);
+    e = check_hip(hipGetLastError(), "batch bucket count launch");
+    if (e) return e;
+  }
+  {
+    StageTimer t(ST_SCAN, s);
+    const int nb2 = (d.max_nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    hipLaunchKernelGGL(k_col_scan_jobs, dim3((d.max_nkeys + COL_NT - 1) / COL_NT, nj), dim3(COL_NT), 0, s, jobs);
+    hipLaunchKernelGGL(k_scan_local_jobs, dim3(nb2, nj), dim3(SCAN_NT), 0, s, jobs);
+    hipLaunchKernelGGL(k_scan_top_jobs, dim3(1, nj), dim3(SCAN_NT), 0, s, jobs);
+    hipLaunchKernelGGL(k_scan_add_jobs, dim3(nb2, nj), dim3(SCAN_NT), 0, s, jobs);
+    int e = check_hip(hipGetLastError(), "batch scan launch");
+    if (e) return e;
+  }
+  StageTimer t(ST_SORT_PLACE, s);
+  const size_t lds = dest_lds(d.max_nkeys);
+  int e = allow_lds(k_bucket_place_jobs, lds);
+  if (e) return e;
+  hipLaunchKernelGGL(k_bucket_place_jobs, dim3(d.max_nblk, nj), dim3(CNT_NT), lds, s, jobs);
+  return check_hip(hipGetLastError(), "batch bucket place launch");
+}
+
+int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s) {
+  StageTimer t(ST_MASS, s);
+  hipLaunchKernelGGL(k_mass_finalize_jobs, dim3(1, njobs), dim3(ATM_NT), 0, s, jobs);
+  return check_hip(hipGetLastError(), "batch mass finalize launch");
 }
 
 }  // namespace pk
