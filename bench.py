@@ -175,6 +175,111 @@ def cpu_baseline(pos_host, n, box, edges, shot):
                       f"({dt:.1f} s; FFT on {cores} threads, scatter and binning single-threaded NumPy)"}
 
 
+BATCH_MOCKS = 256
+BATCH_NS = (64, 96, 128, 192, 256)
+
+
+def make_batch_mocks(dev, count=BATCH_MOCKS, seed=3000):
+    """Covariance set of uneven lognormal mocks: L ~ U[500, 1500] Mpc/h, n ~ {64..256},
+    N_p log-uniform in [1e5, 4e6], CIC for even i and TSC for odd i, k_F bins up to k_N."""
+    cats = []
+    for i in range(count):
+        rng = np.random.default_rng(seed + i)
+        box = float(rng.uniform(500.0, 1500.0))
+        n = int(rng.choice(BATCH_NS))
+        npart = float(np.exp(rng.uniform(np.log(1e5), np.log(4e6))))
+        pos = make_catalogue(n, box, npart / box ** 3, dev, seed=seed + i)
+        kf = 2 * np.pi / box
+        cats.append(dict(x=pos[:, 0], y=pos[:, 1], z=pos[:, 2], box_size=box, n_bins=n,
+                         mas="CIC" if i % 2 == 0 else "TSC", k_edges=kf * np.arange(1, n // 2 + 1),
+                         shot_noise=box ** 3 / pos.shape[0]))
+    return cats
+
+
+def bench_batched(dev, reps=5):
+    """256 uneven mocks: plain one-call-per-catalogue estimate() vs resident graph-replayed
+    batches with autotuned tiles.  Both produce the same P_l (checked here)."""
+    import torch
+    from jax_powspec_amd import batch, correlations
+    log("batched: generating mocks ...")
+    cats = make_batch_mocks(dev)
+    npart = sum(int(c["x"].numel()) for c in cats)
+    torch.cuda.synchronize()
+
+    def plain_pass():
+        return [correlations.estimate(c["x"], c["y"], c["z"], box_size=c["box_size"], n_bins=c["n_bins"],
+                                      k_edges=c["k_edges"], mas=c["mas"], shot_noise=c["shot_noise"], full=True)
+                for c in cats]
+
+    plain_pass()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ref = plain_pass()
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / reps
+    log(f"batched: plain {plain_s * 1e3:.1f} ms for {len(cats)} mocks")
+    groups = {}
+    for i, c in enumerate(cats):
+        groups.setdefault(c["n_bins"], []).append(i)
+    tiles, tune = {}, {}
+    batches = []
+    for n, idx in sorted(groups.items()):
+        sub = [cats[i] for i in idx]
+        best, timings = batch.autotune_tile(n, sub, reps=3, device=dev)
+        tiles[n] = best[1]
+        tune[n] = {str(k): round(v, 3) for k, v in timings.items()}
+        b = batch.BatchEstimator(n, len(sub), max(int(c["x"].numel()) for c in sub),
+                                 max(len(c["k_edges"]) - 1 for c in sub), tile=best, graph=True, device=dev)
+        b.set(sub)
+        batches.append((b, idx))
+    bl = [b for b, _ in batches]
+    for _ in range(2):   # plain launches, then capture
+        batch.run_concurrent(bl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for b in bl:
+            b.run()
+    torch.cuda.synchronize()
+    batch_serial_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run_concurrent(bl)
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for b, idx in batches:
+            b.set([cats[i] for i in idx])
+        batch.run_concurrent(bl)
+    torch.cuda.synchronize()
+    batch_set_s = (time.perf_counter() - t0) / reps
+    # same results as the plain path (same tiles would be bit-identical up to FP64 sum order)
+    worst = 0.0
+    counts_equal = True
+    for b, idx in batches:
+        for k, i in enumerate(idx):
+            _, pk_b, nm_b, ex_b = b.result(k, full=True)
+            _, pk_p, nm_p, ex_p = ref[i]
+            counts_equal &= bool(torch.equal(ex_b["counts"], ex_p["counts"]))
+            s_b, s_p = ex_b["sums"], ex_p["sums"]
+            scale = s_p.abs().amax(dim=0, keepdim=True).clamp_min(1e-300)
+            worst = max(worst, float(((s_b - s_p).abs() / scale).max()))
+    log(f"batched: {batch_s * 1e3:.1f} ms per pass ({plain_s / batch_s:.2f}x), tiles {tiles}")
+    return {"mocks": len(cats), "particles": npart, "mesh_sides": {str(n): len(v) for n, v in sorted(groups.items())},
+            "plain_ms": round(plain_s * 1e3, 3), "batched_ms": round(batch_s * 1e3, 3),
+            "batched_with_refill_ms": round(batch_set_s * 1e3, 3),
+            "batched_groups_in_sequence_ms": round(batch_serial_s * 1e3, 3),
+            "plain_estimates_per_s": round(len(cats) / plain_s, 1),
+            "batched_estimates_per_s": round(len(cats) / batch_s, 1),
+            "speedup": round(plain_s / batch_s, 3), "autotuned_ty": {str(k): v for k, v in tiles.items()},
+            "autotune_ms_per_batch": tune, "max_rel_sum_diff_vs_plain": worst, "counts_equal": counts_equal,
+            "note": "plain = correlations.estimate() per mock; batched = one graph-replayed launch sequence "
+                    "per mesh side over resident slots, the mesh-side groups on concurrent streams; "
+                    "inputs resident in HBM"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +290,8 @@ def main():
     ap.add_argument("--tile", type=str, default=None, help="1,ty,0 | direct")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (HBM counter calibration)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-mock batched comparison")
+    ap.add_argument("--only-batched", action="store_true", help="run only the batched comparison")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the calibrated HBM traffic summary of two rocprofv3 --pmc runs and exit")
     ap.add_argument("--pmc-command", default="", help="command line recorded in the --pmc-summary header")
@@ -212,6 +319,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.only_batched:
+        print(json.dumps({"batched": bench_batched(dev)}), flush=True)
+        return
     log("generating catalogue ...")
     pos = make_catalogue(N_MESH, BOX, NBAR, dev)
     npart = pos.shape[0]
@@ -277,6 +387,10 @@ def main():
         roof_path = path_bytes / (ms * 1e-3) / 1e9
         log(f"whole-path algorithmic bytes {path_bytes / 1e9:.3f} GB -> {roof_path:.0f} GB/s "
             f"({roof_path / HBM_PEAK_GBS:.3f} of peak)")
+    batched = None
+    if not args.no_batched:
+        del est
+        batched = bench_batched(dev)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -298,6 +412,7 @@ def main():
                                "P_0,2,4 in 256 k-bins", "mesh": N_MESH, "particles": npart, "bins": NBINS},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

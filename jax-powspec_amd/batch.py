@@ -27,7 +27,7 @@ import torch
 from . import _device, _lib, binning
 from .correlations import _window_p
 
-__all__ = ["BatchEstimator", "powspec_batched", "autotune_tile", "tile_candidates", "TUNED_TILES"]
+__all__ = ["BatchEstimator", "powspec_batched", "autotune_tile", "tile_candidates", "run_concurrent", "TUNED_TILES"]
 
 # n -> (1, ty, 0) chosen by autotune_tile (per process)
 TUNED_TILES: dict = {}
@@ -202,6 +202,23 @@ class BatchEstimator:
         return [self.result(i, full) for i in range(len(self.jobs))]
 
 
+def run_concurrent(batches, *, stream=None):
+    """Run several batches (e.g. one per mesh side) concurrently, each on its own stream,
+    all ordered after the caller's stream and joined back into it."""
+    if not batches:
+        return
+    dev = batches[0].device
+    cur = stream if stream is not None else torch.cuda.current_stream(dev)
+    lib = _lib.load()
+    with torch.cuda.device(dev):
+        for b in batches:
+            b._stream.wait_stream(cur)
+        for b in batches:
+            _lib.check("pk_batch_run", lib.pk_batch_run(b.handle, b._stream.cuda_stream))
+        for b in batches:
+            cur.wait_stream(b._stream)
+
+
 def autotune_tile(n: int, catalogues, *, reps: int = 3, device=None, graph: bool = True):
     """Time every compiled row-bundle height on ``catalogues`` (same n) and keep the fastest.
 
@@ -252,7 +269,7 @@ def powspec_batched(catalogues, *, tile="auto", graph: bool = True, device=None,
     for i, c in enumerate(cats):
         groups[int(c["n_bins"])].append(i)
     out = [None] * len(cats)
-    keep = []
+    runs = []
     for n, idx in sorted(groups.items()):
         sub = [cats[i] for i in idx]
         t = tile
@@ -266,9 +283,11 @@ def powspec_batched(catalogues, *, tile="auto", graph: bool = True, device=None,
             part = jobs[s0:s0 + cap]
             b = BatchEstimator(n, len(part), max(1, max(j.np for j in part)), max(j.nb for j in part), tile=t,
                                graph=graph, device=device)
-            b.set(part).run()
-            for k, i in enumerate(idx[s0:s0 + cap]):
-                out[i] = b.result(k, full)
-            keep.append(b)
+            b.set(part)
+            runs.append((b, idx[s0:s0 + cap]))
+    run_concurrent([b for b, _ in runs])
+    for b, idx in runs:
+        for k, i in enumerate(idx):
+            out[i] = b.result(k, full)
     torch.cuda.synchronize(device)
     return out

@@ -96,13 +96,30 @@ __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const floa
   constexpr int NTILES = (N / 2 + 1) * N / C;
   __shared__ float2 buf[C * CS];
   __shared__ float2 tw[N];
+  constexpr int PER = (C * N + NT - 1) / NT;
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  // the next tile's rows are loaded into registers while this tile is transformed
+  float2 pre[PER];
+  auto load = [&](int tile) {
+    const float2* row = spec + (size_t)tile * C * N;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = tid + k * NT;
+      if (i < C * N) pre[k] = row[i];
+    }
+  };
+  if (bx < NTILES) load(bx);
   for (int tile = bx; tile < NTILES; tile += nbx) {
-    float2* row = spec + (size_t)tile * C * N;
-    for (int i = tid; i < C * N; i += NT) buf[(i / N) * CS + (i % N)] = row[i];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = tid + k * NT;
+      if (i < C * N) buf[(i / N) * CS + (i % N)] = pre[k];
+    }
     __syncthreads();
+    if (tile + nbx < NTILES) load(tile + nbx);
     lds_fft<N, C, 1, CS, NT, false, 1>(buf, tw);
+    float2* row = spec + (size_t)tile * C * N;
     for (int i = tid; i < C * N; i += NT) row[i] = buf[(i / N) * CS + (i % N)];
     __syncthreads();
   }
@@ -159,6 +176,63 @@ __device__ __forceinline__ int find_bin(int m, const int* __restrict__ thr, cons
   return b;
 }
 
+// Wave64 cross-lane moves (DPP on the VALU, no LDS traffic).  Lanes without a source
+// lane (or in rows the row mask leaves out) read 0.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, true);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v)));
+}
+constexpr int DPP_ROW_SHR = 0x110;   // + shift (1..15), within rows of 16 lanes
+constexpr int DPP_WAVE_SHR1 = 0x138; // lane l reads lane l - 1
+constexpr int DPP_WAVE_SHL1 = 0x130; // lane l reads lane l + 1
+constexpr int DPP_BCAST15 = 0x142;   // a row reads lane 15 of the row before (rows 1, 3 here)
+constexpr int DPP_BCAST31 = 0x143;   // a row reads lane 31 (rows 2, 3)
+
+// one step of a segmented inclusive scan: (v, f) <- (v + v_src, f_src) where this lane's
+// run has not started yet (f == 0); a lane without a source reads (0, 0) and is unchanged
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step(float& a, float& b, float& c, float& d, float& e, int& f, bool with_e) {
+  const float pa = dpp_move<CTRL, ROWS>(a), pb = dpp_move<CTRL, ROWS>(b), pc = dpp_move<CTRL, ROWS>(c);
+  const float pd = dpp_move<CTRL, ROWS>(d);
+  const float pe = with_e ? dpp_move<CTRL, ROWS>(e) : 0.0f;
+  const int pf = dpp_move<CTRL, ROWS>(f);
+  if (!f) {
+    a += pa;
+    b += pb;
+    c += pc;
+    d += pd;
+    e += pe;
+    f = pf;
+  }
+}
+
+// Sums of the values over each run of lanes with equal `slot` (lane order), left in the
+// run's last lane; returns true in that lane.  A segmented inclusive scan over the 64
+// lanes: shifts 1, 2, 4, 8 inside rows of 16, then row 0's total into row 1 and row 2's
+// into row 3 (bcast15), then rows 0-1's total into rows 2 and 3 (bcast31).
+// Float partial sums of at most 64 terms (depth-6 tree: relative error <= 6 ulp of the
+// run's absolute sum).  Must be called by all 64 lanes.
+__device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, float& c, float& d, float& e, bool with_e) {
+  const int lane = threadIdx.x & 63;
+  // every cross-lane move runs with all 64 lanes active (a lane the exec mask leaves out
+  // reads as 0 to its neighbours), so none sits behind a short-circuit or a branch
+  const int prev = dpp_move<DPP_WAVE_SHR1>(slot + 1) - 1;   // lane 0 reads -1 (no source)
+  int f = (lane == 0 || prev != slot) ? 1 : 0;               // this lane starts a run
+  const int next_starts = dpp_move<DPP_WAVE_SHL1>(f);
+  const bool tail = (lane == 63) | (next_starts != 0);
+  seg_step<DPP_ROW_SHR + 1, 0xf>(a, b, c, d, e, f, with_e);
+  seg_step<DPP_ROW_SHR + 2, 0xf>(a, b, c, d, e, f, with_e);
+  seg_step<DPP_ROW_SHR + 4, 0xf>(a, b, c, d, e, f, with_e);
+  seg_step<DPP_ROW_SHR + 8, 0xf>(a, b, c, d, e, f, with_e);
+  seg_step<DPP_BCAST15, 0xa>(a, b, c, d, e, f, with_e);
+  seg_step<DPP_BCAST31, 0xc>(a, b, c, d, e, f, with_e);
+  return tail;
+}
+
 constexpr int HB_LDS = 256;   // bins held in LDS; above this the per-block slice lives in HBM
 constexpr int NACC = 5;       // S0, S2, S4, count, sum sqrt(m)
 
@@ -167,6 +241,16 @@ constexpr int NACC = 5;       // S0, S2, S4, count, sum sqrt(m)
 // Each persistent block accumulates its own histogram and writes it to
 // partials[block][nb][NACC] once at the end.
 // ------------------------------------------------------------------------------------
+// LDS of the binning x pass and the waves per SIMD it leaves room for; the register
+// budget is held to that so registers never cost a resident block
+template <int N> struct XBinLds {
+  static constexpr size_t TILE = (size_t)N * (FFTCfg<N>::CX + 1) * 8;
+  static constexpr size_t HIST = (size_t)(HB_LDS + 1) * 4 + 4 * HB_LDS * 8 + HB_LDS * 4;
+  template <bool LH> static constexpr int wpe() {
+    return cmax(1, cmin(4, (int)((160 * 1024) / (TILE + (LH ? HIST : 64))) * (FFTCfg<N>::NTX / 64) / 4));
+  }
+};
+
 template <int N, bool LH>
 __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __restrict__ twg, int bx, int nbx) {
   const float2* __restrict__ spec = j.spec;
@@ -180,12 +264,14 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   constexpr int YT = N / C;
   constexpr int NTILES = (N / 2 + 1) * YT;
   constexpr int H = N / 2;
-  // LDS: the tile, the S_0,2,4 histogram, counts and thresholds (<= 80 KB at N = 512 so two
-  // blocks share a CU); twiddles, window and isqrt-start tables are read through the caches
+  // LDS: the tile, the S_0,2,4 / count / sum-k histogram and the thresholds (<= 80 KB at
+  // N = 512 so two blocks share a CU); twiddles, window and isqrt-start tables are read
+  // through the caches
   __shared__ float2 buf[N * PS];
   __shared__ int thr_s[LH ? HB_LDS + 1 : 1];
   __shared__ double hs[LH ? 3 * HB_LDS : 1];
   __shared__ unsigned int hc[LH ? HB_LDS : 1];
+  __shared__ double hk[LH ? HB_LDS : 1];
   const float2* tw = twg;
   const float* wt = wtab;
   const int* jst = jst_g;
@@ -194,8 +280,10 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
     for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0.0;
-    for (int i = tid; i < HB_LDS; i += NT) hc[i] = 0u;
-    for (int i = tid; i < nb; i += NT) gpart[(size_t)i * NACC + 4] = 0.0;   // sum sqrt(m) stays in HBM
+    for (int i = tid; i < HB_LDS; i += NT) {
+      hc[i] = 0u;
+      hk[i] = 0.0;
+    }
   } else {
     for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;
   }
@@ -210,49 +298,63 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
     const float wz = wt[kz];
     const int kz2 = kz * kz;
-    // pair tasks: |kx| = q in [0, N/2], column c; entries q and N-q share m and mu
-    for (int i = tid; i < (H + 1) * C; i += NT) {
-      const int c = i % C, q = i / C;
-      const int y = yt * C + c;
-      const int ky = y <= H ? y : y - N;
-      const int m = q * q + ky * ky + kz2;
-      const int slot = find_bin(m, thr, jst, nb);
-      if (slot < 0) continue;
-      const float2 a = buf[q * PS + c];
-      float p2 = a.x * a.x + a.y * a.y;
-      int cnt = 1;
-      if (q != 0 && q != H) {
-        const float2 bb = buf[(N - q) * PS + c];
-        p2 += bb.x * bb.x + bb.y * bb.y;
-        cnt = 2;
+    // pair tasks: |kx| = q in [0, N/2], column c; entries q and N-q share m and mu.  The
+    // loop is wave-uniform (idle lanes carry slot -1) so the wave can combine lanes that
+    // share a bin before touching the histogram.
+    for (int i0 = 0; i0 < (H + 1) * C; i0 += NT) {
+      const int i = i0 + tid;
+      int slot = -1;
+      float p2 = 0.0f, v2 = 0.0f, v4 = 0.0f, vc = 0.0f, vk = 0.0f;
+      if (i < (H + 1) * C) {
+        const int c = i % C, q = i / C;
+        const int y = yt * C + c;
+        const int ky = y <= H ? y : y - N;
+        const int m = q * q + ky * ky + kz2;
+        slot = find_bin(m, thr, jst, nb);
+        if (slot >= 0) {
+          const float2 a = buf[q * PS + c];
+          p2 = a.x * a.x + a.y * a.y;
+          int cnt = 1;
+          if (q != 0 && q != H) {
+            const float2 bb = buf[(N - q) * PS + c];
+            p2 += bb.x * bb.x + bb.y * bb.y;
+            cnt = 2;
+          }
+          p2 *= wt[q] * wt[y] * wz;
+          // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
+          //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
+          // so L2 is exactly 0 where 3 kz^2 = m.  The k = 0 mode's power is added by the
+          // finalize from the FP64 sum of the mesh (mu = 0 there, :37).
+          if (m > 0) {
+            const float rm = 1.0f / (float)m;
+            const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
+            const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
+            const float l4 = 0.125f * (float)num4 * rm * rm;
+            v2 = p2 * l2;
+            v4 = p2 * l4;
+          } else {
+            p2 = 0.0f;
+          }
+          vc = (float)cnt;
+          vk = want_k ? (float)cnt * sqrtf((float)m) : 0.0f;
+        }
       }
-      p2 *= wt[q] * wt[y] * wz;
-      // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
-      //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
-      // so L2 is exactly 0 where 3 kz^2 = m.  The k = 0 mode's power is added by the
-      // finalize from the FP64 sum of the mesh (mu = 0 there, :37).
-      float l2 = 0.0f, l4 = 0.0f;
-      if (m > 0) {
-        const float rm = 1.0f / (float)m;
-        l2 = 0.5f * (float)(3 * kz2 - m) * rm;
-        const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
-        l4 = 0.125f * (float)num4 * rm * rm;
-      } else {
-        p2 = 0.0f;
-      }
+      // runs of equal slot in lane order -> one histogram update per run (its last lane)
+      const bool tail = wave_run_sums(slot, p2, v2, v4, vc, vk, want_k);
+      if (!tail || slot < 0) continue;
       if constexpr (LH) {
         atomicAdd(&hs[slot], (double)p2);
-        atomicAdd(&hs[HB_LDS + slot], (double)(p2 * l2));
-        atomicAdd(&hs[2 * HB_LDS + slot], (double)(p2 * l4));
-        atomicAdd(&hc[slot], (unsigned)cnt);
-        if (want_k) atomicAdd(gpart + (size_t)slot * NACC + 4, (double)cnt * (double)sqrtf((float)m));
+        atomicAdd(&hs[HB_LDS + slot], (double)v2);
+        atomicAdd(&hs[2 * HB_LDS + slot], (double)v4);
+        atomicAdd(&hc[slot], (unsigned)vc);
+        if (want_k) atomicAdd(&hk[slot], (double)vk);
       } else {
         double* g = gpart + (size_t)slot * NACC;
         atomicAdd(g + 0, (double)p2);
-        atomicAdd(g + 1, (double)(p2 * l2));
-        atomicAdd(g + 2, (double)(p2 * l4));
-        atomicAdd(g + 3, (double)cnt);
-        if (want_k) atomicAdd(g + 4, (double)cnt * (double)sqrtf((float)m));
+        atomicAdd(g + 1, (double)v2);
+        atomicAdd(g + 2, (double)v4);
+        atomicAdd(g + 3, (double)vc);
+        if (want_k) atomicAdd(g + 4, (double)vk);
       }
     }
     __syncthreads();
@@ -264,18 +366,20 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       g[1] = hs[HB_LDS + i];
       g[2] = hs[2 * HB_LDS + i];
       g[3] = (double)hc[i];
+      g[4] = hk[i];
     }
   }
 }
 
 template <int N, bool LH>
-__global__ void __launch_bounds__(FFTCfg<N>::NTX) k_xpass_bin(Job j, const float2* __restrict__ twg) {
+__global__ void __launch_bounds__(FFTCfg<N>::NTX) __attribute__((amdgpu_waves_per_eu(XBinLds<N>::template wpe<LH>())))
+k_xpass_bin(Job j, const float2* __restrict__ twg) {
   xpass_bin_body<N, LH>(j, twg, blockIdx.x, gridDim.x);
 }
 
 template <int N, bool LH>
-__global__ void __launch_bounds__(FFTCfg<N>::NTX) k_xpass_bin_jobs(const Job* __restrict__ jobs,
-                                                                   const float2* __restrict__ twg) {
+__global__ void __launch_bounds__(FFTCfg<N>::NTX) __attribute__((amdgpu_waves_per_eu(XBinLds<N>::template wpe<LH>())))
+k_xpass_bin_jobs(const Job* __restrict__ jobs, const float2* __restrict__ twg) {
   const Job& j = jobs[blockIdx.y];
   if (!j.active) return;
   xpass_bin_body<N, LH>(j, twg, blockIdx.x, gridDim.x);
