@@ -140,7 +140,30 @@ int pk_estimate(pk_plan* plan,
                 double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
                 double* stats, void* workspace, size_t workspace_bytes, void* stream);
 
-/* --- batches: many catalogues of one mesh side per launch --------------------------
+/* --- backward (reverse-mode derivative of an estimate) ------------------------------
+ * Replaces jax.value_and_grad through cic_mas_vec + powspec_vec (tests/lognormal.py:101):
+ * given dLoss/dP_l (grad_pk[nb][3], FP64) of a pk_estimate call, returns dLoss/d position
+ * (grad_pos[np][3], float32, physical units) and optionally dLoss/d weight (grad_w[np]).
+ * The particle, tile, bin and window arguments must be those of the forward call;
+ * fwd_workspace is that call's workspace, unchanged since (it holds the spectrum after the
+ * y pass), stats and counts its outputs.  The shot-noise term is constant and has no
+ * derivative; the k = 0 mode is 0 in the forward and contributes none.  Through the
+ * density contrast, the mean density's dependence on the particles is included.
+ * No reference counterpart exists for the hand-written adjoint itself: the reference
+ * differentiates its JAX code automatically.
+ */
+int pk_estimate_backward_workspace_size(pk_plan* plan, int64_t np, int nb, size_t* bytes);
+int pk_estimate_backward(pk_plan* plan,
+                         const float* x, int64_t sx, const float* y, int64_t sy, const float* z, int64_t sz,
+                         const float* w, int64_t sw, int64_t np,
+                         float xmin, float ymin, float zmin, double box_size,
+                         int stencil, int wrap_mode, const int* tile,
+                         const int32_t* thr, const int32_t* jstart, int nb, int window_p,
+                         const void* fwd_workspace, size_t fwd_workspace_bytes,
+                         const double* stats, const double* grad_pk, const int64_t* counts,
+                         float* grad_pos, float* grad_w, void* workspace, size_t workspace_bytes, void* stream);
+
+/* --- batches: many catalogues/* --- batches: many catalogues of one mesh side per launch --------------------------
  * No reference counterpart: the reference estimates one catalogue per call.  A batch
  * runs up to max_jobs estimates (each as pk_estimate: its own particles, stencil, window,
  * bins, shot noise and outputs) with one launch per stage for all of them: blockIdx.y
@@ -191,7 +214,7 @@ int pk_batch_destroy(pk_batch* batch);
  * pk_timing_read synchronises those events, returns total ms and launch counts per stage
  * (arrays of PK_NUM_STAGES) and clears the record.
  */
-#define PK_NUM_STAGES 12
+#define PK_NUM_STAGES 14
 int pk_timing_enable(int on);
 int pk_timing_read(double* ms, int64_t* launches, int nstages);
 const char* pk_stage_name(int stage);

@@ -23,7 +23,7 @@ WRAP_VEC_TRUE, WRAP_VEC_FALSE, WRAP_SCAN_TRUE, WRAP_SCAN_FALSE = 0, 1, 2, 3
 FLAG_ZERO_DC = 1
 FLAG_KMEAN = 2
 STATS_LEN = 4
-NUM_STAGES = 12
+NUM_STAGES = 14
 
 # (name, restype, argtypes) for every symbol declared in include/pk_api.h
 _vp = C.c_void_p
@@ -64,6 +64,9 @@ SIGNATURES = [
     ("pk_estimate_workspace_size", _i, [_vp, _i64, _i, _vp, _i, _psz]),
     ("pk_estimate", _i, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _f, _f, _f, _d, _i, _i, _vp,
                          _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    ("pk_estimate_backward_workspace_size", _i, [_vp, _i64, _i, _psz]),
+    ("pk_estimate_backward", _i, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _f, _f, _f, _d, _i, _i, _vp,
+                                  _vp, _vp, _i, _i, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     ("pk_batch_workspace_size", _i, [_vp, _i, _i64, _i, _vp, _psz]),
     ("pk_batch_create", _i, [_vp, _i, _i64, _i, _vp, _i, _vp, _sz, C.POINTER(_vp)]),
     ("pk_batch_set_jobs", _i, [_vp, _i, _vp, _vp]),

@@ -56,7 +56,8 @@ std::vector<Rec> g_recs;
 thread_local hipEvent_t g_open[PK_NUM_STAGES] = {};
 const char* g_stage_names[PK_NUM_STAGES] = {"sort_count", "scan",   "sort_place",  "deposit",
                                             "atomic",     "mass",   "zpass",       "ypass",
-                                            "xpass_bin",  "finalize", "xpass_plain", "fill"};
+                                            "xpass_bin",  "finalize", "xpass_plain", "fill",
+                                            "adjoint_fft", "gather"};
 }  // namespace
 
 void timing_suspend(bool on);
@@ -439,6 +440,86 @@ int pk_estimate(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t 
   e = mass_finalize(j, s);
   if (e) return e;
   return finish_powspec(p, j, s);
+}
+
+namespace {
+struct BackwardWS {
+  float2* bspec;
+  float* gmesh;
+  float* coef;
+  void* gather;
+  size_t gather_bytes;
+  size_t bytes;
+};
+
+BackwardWS carve_backward(int n, int64_t np, int nb, void* base) {
+  BackwardWS w{};
+  char* b = static_cast<char*>(base);
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* q = b ? b + off : nullptr;
+    off += align_up(bytes);
+    return q;
+  };
+  w.bspec = static_cast<float2*>(take((size_t)(n / 2 + 1) * n * n * sizeof(float2)));
+  w.gmesh = static_cast<float*>(take((size_t)n * n * n * sizeof(float)));
+  w.coef = static_cast<float*>(take((size_t)nb * 3 * sizeof(float)));
+  w.gather_bytes = gather_workspace_bytes(np);
+  w.gather = take(w.gather_bytes);
+  w.bytes = off;
+  return w;
+}
+}  // namespace
+
+int pk_estimate_backward_workspace_size(pk_plan* p, int64_t np, int nb, size_t* bytes) {
+  int e = check_plan(p);
+  if (e) return e;
+  if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
+  if (nb < 1) return set_error(PK_E_INVALID, "need at least one bin (nb=%d)", nb);
+  if (np < 0 || np > 0x7fffffffLL) return set_error(PK_E_INVALID, "particle count out of range");
+  *bytes = carve_backward(p->n, np, nb, nullptr).bytes;
+  return PK_OK;
+}
+
+int pk_estimate_backward(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t sy, const float* z,
+                         int64_t sz, const float* w, int64_t sw, int64_t np, float xmin, float ymin, float zmin,
+                         double box_size, int stencil, int wrap_mode, const int* tile, const int32_t* thr,
+                         const int32_t* jstart, int nb, int window_p, const void* fwd_workspace,
+                         size_t fwd_workspace_bytes, const double* stats, const double* grad_pk,
+                         const int64_t* counts, float* grad_pos, float* grad_w, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+  int e = check_plan(p);
+  if (e) return e;
+  if (nb < 1 || !thr || !jstart) return set_error(PK_E_INVALID, "bad bin tables");
+  if (!fwd_workspace || !stats || !grad_pk || !counts || !grad_pos)
+    return set_error(PK_E_INVALID, "null forward workspace / stats / grad_pk / counts / grad_pos");
+  if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
+  ScatterArgs a{};
+  e = make_args(&a, x, sx, y, sy, z, sz, w, sw, np, xmin, ymin, zmin, box_size, p->n, stencil, wrap_mode);
+  if (e) return e;
+  EstimateWS fw{};
+  TileGeom g{};
+  bool direct = false;
+  e = carve_estimate(p, np, stencil, tile, nb, const_cast<void*>(fwd_workspace), &fw, &g, &direct);
+  if (e) return e;
+  if (fwd_workspace_bytes < fw.bytes)
+    return set_error(PK_E_WORKSPACE, "forward workspace %zu < %zu bytes", fwd_workspace_bytes, fw.bytes);
+  BackwardWS bw = carve_backward(p->n, np, nb, workspace);
+  if (workspace_bytes < bw.bytes)
+    return set_error(PK_E_WORKSPACE, "backward workspace %zu < %zu bytes", workspace_bytes, bw.bytes);
+  hipStream_t s = as_stream(stream);
+  const int n = p->n;
+  const double h = box_size / ((double)n * (double)n);
+  e = adjoint_coefficients(grad_pk, counts, nb, h * h * h, bw.coef, s);
+  if (e) return e;
+  Job j{};
+  j.active = 1;
+  set_bin_fields(j, p, fw.spec, thr, jstart, nb, window_p, PK_FLAG_ZERO_DC, box_size, 0.0, nullptr, nullptr, nullptr,
+                 nullptr, nullptr);
+  j.scale_stats = fw.fused ? stats : nullptr;   // fused: the spectrum is of rho - c, delta_k = S spec_k
+  e = fft_adjoint(n, j, p->tw, bw.coef, bw.bspec, bw.gmesh, s);
+  if (e) return e;
+  return gather_gradients(a, bw.gmesh, stats, grad_pos, grad_w, bw.gather, bw.gather_bytes, s);
 }
 
 // ------------------------------------------------------------------------------------

@@ -89,7 +89,8 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
 // ------------------------------------------------------------------------------------
 // y pass: contiguous rows of N complex, in place
 // ------------------------------------------------------------------------------------
-template <int N>
+// CONJ: conj on load and store, i.e. the unnormalised inverse transform (e^{+i})
+template <int N, bool CONJ = false>
 __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const float2* __restrict__ twg, int bx, int nbx) {
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CY, NT = Cf::NTY, CS = N + 1;
@@ -106,7 +107,10 @@ __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const floa
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = tid + k * NT;
-      if (i < C * N) pre[k] = row[i];
+      if (i < C * N) {
+        pre[k] = row[i];
+        if constexpr (CONJ) pre[k].y = -pre[k].y;
+      }
     }
   };
   if (bx < NTILES) load(bx);
@@ -120,7 +124,11 @@ __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const floa
     if (tile + nbx < NTILES) load(tile + nbx);
     lds_fft<N, C, 1, CS, NT, false, 1>(buf, tw);
     float2* row = spec + (size_t)tile * C * N;
-    for (int i = tid; i < C * N; i += NT) row[i] = buf[(i / N) * CS + (i % N)];
+    for (int i = tid; i < C * N; i += NT) {
+      float2 v = buf[(i / N) * CS + (i % N)];
+      if constexpr (CONJ) v.y = -v.y;
+      row[i] = v;
+    }
     __syncthreads();
   }
 }
@@ -453,6 +461,138 @@ __global__ void __launch_bounds__(FIN_NT) k_bin_finalize_jobs(const Job* __restr
 }
 
 // ------------------------------------------------------------------------------------
+// Adjoint of the binning (backward of P_l(k) with respect to the density contrast).
+//
+// Forward: P_lb = c_lb sum_{k in b} W_k L_l(mu_k) |delta_k|^2 over the stored half
+// spectrum, c_lb = norm (2l+1) / N_b.  With g_lb = dLoss/dP_lb and
+// G_k = W_k sum_l g_lb c_lb L_l(mu_k), dLoss/d delta(x) = Re sum_{half} A_k e^{+i k.x} with
+// A_k = 2 G_k delta_k (delta_k = S spec_k; S = stats[1] when the spectrum is of rho - c).
+// Three passes mirror the forward ones: x (forward FFT, times 2 G S, inverse FFT),
+// y (inverse), z (half-spectrum to real rows).
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTX)
+k_xpass_adjoint(Job j, const float2* __restrict__ twg, const float* __restrict__ coef, float2* __restrict__ bspec) {
+  using Cf = FFTCfg<N>;
+  constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = (N / 2 + 1) * YT;
+  constexpr int H = N / 2;
+  __shared__ float2 buf[N * PS];
+  const float2* __restrict__ spec = j.spec;
+  const float* __restrict__ wt = j.wtab;
+  const int* __restrict__ thr = j.thr;
+  const int* __restrict__ jst = j.jst;
+  const int nb = j.nb;
+  const float sf = j.scale_stats ? (float)j.scale_stats[1] : 1.0f;
+  const int tid = threadIdx.x;
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int kz = tile / YT, yt = tile % YT;
+    const float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
+    for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
+    __syncthreads();
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, twg);
+    const float wz = wt[kz];
+    const int kz2 = kz * kz;
+    for (int i = tid; i < N * C; i += NT) {
+      const int c = i % C, r = i / C;
+      const int y = yt * C + c;
+      const int kx = r <= H ? r : r - N;
+      const int ky = y <= H ? y : y - N;
+      const int m = kx * kx + ky * ky + kz2;
+      float g = 0.0f;
+      if (m > 0) {   // the k = 0 mode is exactly 0 in the forward (contrast applied)
+        const int slot = find_bin(m, thr, jst, nb);
+        if (slot >= 0) {
+          const float rm = 1.0f / (float)m;
+          const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
+          const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
+          const float l4 = 0.125f * (float)num4 * rm * rm;
+          g = wt[r] * wt[y] * wz * (coef[3 * slot] + coef[3 * slot + 1] * l2 + coef[3 * slot + 2] * l4);
+        }
+      }
+      const float sc = 2.0f * g * sf;
+      const float2 v = buf[r * PS + c];
+      buf[r * PS + c] = make_float2(sc * v.x, -sc * v.y);   // conj(A): inverse via the forward FFT
+    }
+    __syncthreads();
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, twg);
+    float2* ob = bspec + (size_t)kz * N * N + (size_t)yt * C;
+    for (int i = tid; i < N * C; i += NT) {
+      const float2 v = buf[(i / C) * PS + (i % C)];
+      ob[(size_t)(i / C) * N + (i % C)] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_inverse(float2* __restrict__ spec,
+                                                                  const float2* __restrict__ twg) {
+  ypass_body<N, true>(spec, twg, blockIdx.x, gridDim.x);
+}
+
+// Half-spectrum rows B[kz][x][y] (kz = 0..N/2, transformed in x and y) -> real rows
+// u(z) = Re B_0 + Re B_{N/2} (-1)^z + sum_{0<k<N/2} Re(B_k e^{2 pi i k z / N}), i.e. the C2R
+// transform of Y_0 = Re B_0, Y_{N/2} = Re B_{N/2}, Y_k = B_k / 2: an N/2-point complex
+// inverse FFT of Z_k = (Y_k + conj Y_{N/2-k}) + i e^{2 pi i k / N} (Y_k - conj Y_{N/2-k}),
+// whose outputs are u(2j) + i u(2j+1).
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTZ)
+k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, float* __restrict__ out) {
+  using Cf = FFTCfg<N>;
+  constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = N * YT;
+  __shared__ float2 buf[C * CS];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int x = tile / YT, yt = tile % YT;
+    for (int i = tid; i < (NC + 1) * C; i += NT) {
+      const int c = i % C, k = i / C;
+      buf[c * CS + k] = bspec[((size_t)k * N + x) * N + (size_t)yt * C + c];
+    }
+    __syncthreads();
+    // pairs (k, NC - k), k = 0..NC/2: each thread reads and rewrites only its own pair
+    // (row[NC] is read by the k = 0 pair and never written), so no barrier is needed
+    for (int i = tid; i < (NC / 2 + 1) * C; i += NT) {
+      const int c = i % C, k = i / C;
+      const int kk = NC - k;
+      float2* row = buf + c * CS;
+      auto Y = [&](int q) {
+        const float2 b = row[q];
+        if (q == 0 || q == NC) return make_float2(b.x, 0.0f);
+        return make_float2(0.5f * b.x, 0.5f * b.y);
+      };
+      const float2 yk = Y(k), ykk = Y(kk);
+      auto zval = [&](const float2 a, const float2 bconj_src, int q) {
+        // (a + conj b) + i e^{2 pi i q / N} (a - conj b), e^{2 pi i q / N} = conj tw[q]
+        const float2 s = make_float2(a.x + bconj_src.x, a.y - bconj_src.y);
+        const float2 d = make_float2(a.x - bconj_src.x, a.y + bconj_src.y);
+        const float2 e = make_float2(tw[q].x, -tw[q].y);
+        const float2 ed = cmul(e, d);
+        return make_float2(s.x - ed.y, s.y + ed.x);
+      };
+      const float2 zk = zval(yk, ykk, k);
+      const float2 zkk = zval(ykk, yk, kk);
+      row[k] = make_float2(zk.x, -zk.y);                 // conj: inverse via the forward FFT
+      if (kk != k && kk < NC) row[kk] = make_float2(zkk.x, -zkk.y);
+    }
+    __syncthreads();
+    lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+    float2* orow = reinterpret_cast<float2*>(out + ((size_t)x * N + (size_t)yt * C) * N);
+    for (int i = tid; i < C * NC; i += NT) {
+      const int c = i / NC, jj = i % NC;
+      const float2 v = buf[c * CS + jj];
+      orow[(size_t)c * NC + jj] = make_float2(v.x, -v.y);   // (u(2j), u(2j+1))
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------
 template <typename K>
@@ -525,6 +665,17 @@ template <int N> struct FFTLaunch {
     else
       hipLaunchKernelGGL((k_xpass_bin<N, false>), dim3(grid), dim3(Cf::NTX), 0, s, j, tw);
     return check_hip(hipGetLastError(), "x pass + bin launch");
+  }
+  static int adjoint(const Job& j, const float2* tw, const float* coef, float2* bspec, float* gmesh, hipStream_t s) {
+    static int gx = 0, gy = 0, gz = 0;
+    if (!gx) gx = persistent_grid(k_xpass_adjoint<N>, Cf::NTX, XTILES);
+    if (!gy) gy = persistent_grid(k_ypass_inverse<N>, Cf::NTY, YTILES);
+    if (!gz) gz = persistent_grid(k_zpass_c2r<N>, Cf::NTZ, ZTILES);
+    StageTimer t(ST_ADJOINT_FFT, s);
+    hipLaunchKernelGGL(k_xpass_adjoint<N>, dim3(gx), dim3(Cf::NTX), 0, s, j, tw, coef, bspec);
+    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, bspec, tw);
+    hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)bspec, tw, gmesh);
+    return check_hip(hipGetLastError(), "adjoint FFT launch");
   }
   static int xpass_bin_jobs(const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s) {
     StageTimer t(ST_XPASS_BIN, s);
@@ -624,6 +775,15 @@ int fft_xpass_bin(int n, const Job& j, const float2* tw, int grid, hipStream_t s
 int fft_xpass_bin_jobs(int n, const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s) {
   switch (n) {
 #define PK_CASE(V) case V: return FFTLaunch<V>::xpass_bin_jobs(jobs, njobs, max_nb, gx, tw, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int fft_adjoint(int n, const Job& j, const float2* tw, const float* coef, float2* bspec, float* gmesh, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::adjoint(j, tw, coef, bspec, gmesh, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }

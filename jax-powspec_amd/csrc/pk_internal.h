@@ -91,6 +91,10 @@ int bin_grid(int n, int nb);   // x-pass blocks of one estimate
 int fft_ypass(int n, const Job& j, const float2* tw, hipStream_t s);
 int fft_xpass_bin(int n, const Job& j, const float2* tw, int grid, hipStream_t s);
 int bin_finalize(const Job& j, int G, hipStream_t s);
+// backward of the binning: per-bin coefficients coef[nb][3] (float) and j's forward
+// spectrum (after the y pass) -> dLoss/d delta on the mesh gmesh [n][n][n]; bspec is a
+// half-spectrum scratch buffer
+int fft_adjoint(int n, const Job& j, const float2* tw, const float* coef, float2* bspec, float* gmesh, hipStream_t s);
 int ypass_grid(int n);
 int fft_ypass_jobs(int n, const Job* jobs, int njobs, int gy, const float2* tw, hipStream_t s);
 int fft_xpass_bin_jobs(int n, const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s);
@@ -118,6 +122,16 @@ struct BatchDims {
 };
 int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s);
 int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s);
+
+// ---- backward of the assignment (pk_adjoint.hip) ------------------------------------------
+// coef[b][l] = grad_pk[b][l] norm (2l+1) / counts[b] (0 for empty bins)
+int adjoint_coefficients(const double* grad_pk, const int64_t* counts, int nb, double norm, float* coef,
+                         hipStream_t s);
+size_t gather_workspace_bytes(int64_t np);
+// dLoss/d position (grad_pos [np][3], physical units) and dLoss/d weight (nullable) from
+// g = dLoss/d delta on the mesh, through delta = rho n^3 / M - 1 (stats: forward M, n^3 / M)
+int gather_gradients(const ScatterArgs& a, const float* gmesh, const double* stats, float* grad_pos, float* grad_w,
+                     void* ws, size_t ws_bytes, hipStream_t s);
 
 // ---- fused assignment + z pass (pk_fused.hip) ----------------------------------------------
 // deposit into LDS rows, subtract stats[3], R2C along z, write the spectrum; per-tile

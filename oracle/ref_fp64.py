@@ -401,3 +401,142 @@ def estimate(x, y, z, w, box_size, n, k_edges=None, stencil="CIC_REF", p=2, shot
     k3D, Pk = res[0], res[1]
     Pk = subtract_shot_noise(Pk, shot_noise)
     return (k3D, Pk) + tuple(res[2:]) + (mesh,)
+
+
+# ----------------------------------------------------------------------------------------
+# reverse-mode derivative of estimate() (checker of the hand-written adjoint)
+# ----------------------------------------------------------------------------------------
+def corner_stencil(stencil, pos, n, mode=WRAP_VEC_TRUE):
+    """Corners of every particle with unit-weight W and dW/dpos per axis (FP64).
+
+    pos: three arrays of scaled positions (x - xmin) * n / box.  Same corners, wrap and
+    drop rules as cic_ref_deposits / cic_mesh / tsc_mesh.  Returns a list of
+    (flat_index, valid, W, (dWx, dWy, dWz)).
+    """
+    pos = [np.asarray(p, F64) for p in pos]
+    ok = np.ones(pos[0].shape, bool)
+    for p in pos:
+        ok &= np.isfinite(p) & (np.abs(p) < 2.0 ** 30)
+    pos = [np.where(ok, p, 0.0) for p in pos]
+    axes = []   # per axis: list of (index, valid, weight, slope) options
+    if stencil == "CIC_REF":
+        for p in pos:
+            i = np.trunc(p).astype(np.int64)
+            d = p - i
+            ip, dz = _plus_index(i, d, n, mode)
+            slope_p = np.where(dz == d, 1.0, 0.0) if mode == WRAP_SCAN_FALSE else np.ones_like(d)
+            ib, vb = _normalise(i, n)
+            ipn, vp = _normalise(ip, n)
+            axes.append(((ib, vb, 1.0 - d, -np.ones_like(d)), (ipn, vp, dz, slope_p)))
+        # src/mas.py:142-151 selections; the :149 corner takes md_y with the index iyp
+        spec = [(0, 0, 0, 0, 0, 0), (1, 0, 0, 1, 0, 0), (0, 1, 0, 0, 1, 0), (0, 0, 1, 0, 0, 1),
+                (1, 1, 0, 1, 1, 0), (1, 0, 1, 1, 0, 1), (0, 1, 1, 0, 0, 1), (1, 1, 1, 1, 1, 1)]
+        out = []
+        for ix, iy, iz, wx, wy, wz in spec:
+            X, Y, Z = axes[0][ix], axes[1][iy], axes[2][iz]
+            WX, WY, WZ = axes[0][wx], axes[1][wy], axes[2][wz]
+            flat = (X[0] * n + Y[0]) * n + Z[0]
+            valid = X[1] & Y[1] & Z[1] & ok
+            W = WX[2] * WY[2] * WZ[2]
+            dW = (WX[3] * WY[2] * WZ[2], WX[2] * WY[3] * WZ[2], WX[2] * WY[2] * WZ[3])
+            out.append((flat, valid, W, dW))
+        return out
+    for p in pos:
+        if stencil == "CIC":
+            f = np.floor(p)
+            d = p - f
+            i = f.astype(np.int64)
+            axes.append([(np.mod(i, n), 1.0 - d, -np.ones_like(d)), (np.mod(i + 1, n), d, np.ones_like(d))])
+        elif stencil == "TSC":
+            c = np.floor(p + 0.5)
+            d = p - c
+            i = c.astype(np.int64)
+            axes.append([(np.mod(i - 1, n), 0.5 * (0.5 - d) ** 2, -(0.5 - d)),
+                         (np.mod(i, n), 0.75 - d * d, -2.0 * d),
+                         (np.mod(i + 1, n), 0.5 * (0.5 + d) ** 2, 0.5 + d)])
+        else:
+            raise ValueError(stencil)
+    out = []
+    for X in axes[0]:
+        for Y in axes[1]:
+            for Z in axes[2]:
+                flat = (X[0] * n + Y[0]) * n + Z[0]
+                out.append((flat, ok, X[1] * Y[1] * Z[1], (X[2] * Y[1] * Z[1], X[1] * Y[2] * Z[1], X[1] * Y[1] * Z[2])))
+    return out
+
+
+def estimate_fp64(x, y, z, w, box_size, n, k_edges, stencil="CIC_REF", p=2, mode=WRAP_VEC_TRUE):
+    """estimate() with FP64 positions (no float32 step): the function whose finite
+    differences pin estimate_backward.  Returns (Pk[nb, 3] before shot noise, counts, mesh)."""
+    n = int(n)
+    inv = n / float(box_size)
+    pos = [np.asarray(c, F64) * inv for c in (x, y, z)]
+    w = np.ones(pos[0].shape) if w is None else np.asarray(w, F64)
+    rho = np.zeros(n ** 3)
+    for flat, valid, W, _ in corner_stencil(stencil, pos, n, mode):
+        rho += np.bincount(flat[valid], weights=(w * W)[valid], minlength=n ** 3)
+    mesh = rho.reshape(n, n, n)
+    delta = contrast(mesh)
+    if k_edges is None:
+        res = powspec_vec_fundamental(delta, box_size, p=p, zero_dc=True, full=True)
+    else:
+        res = powspec_vec(delta, box_size, k_edges, p=p, zero_dc=True, full=True)
+    return res[1], res[3]["counts"], mesh
+
+
+def estimate_backward(x, y, z, w, box_size, n, k_edges, grad_pk, stencil="CIC_REF", p=2, mode=WRAP_VEC_TRUE):
+    """d(sum grad_pk * Pk)/d(positions, weights) of estimate_fp64, by the chain rule.
+
+    P_lb = c_lb sum_{k in b} W_k L_l |delta_k|^2 (half spectrum), delta = rho n^3 / M - 1,
+    rho = sum_i w_i W(x - p_i).  Returns (grad_pos[N, 3], grad_w[N]).
+    """
+    n = int(n)
+    inv = n / float(box_size)
+    pos = [np.asarray(c, F64) * inv for c in (x, y, z)]
+    w = np.ones(pos[0].shape) if w is None else np.asarray(w, F64)
+    corners = corner_stencil(stencil, pos, n, mode)
+    rho = np.zeros(n ** 3)
+    for flat, valid, W, _ in corners:
+        rho += np.bincount(flat[valid], weights=(w * W)[valid], minlength=n ** 3)
+    M = rho.sum()
+    S = n ** 3 / M
+    delta = (rho * S - 1.0).reshape(n, n, n)
+    dk = np.fft.rfftn(delta)
+    dk[0, 0, 0] = 0.0
+    kx, ky, kz = half_spectrum_grids(n)
+    m = (kx * kx + ky * ky + kz * kz).astype(np.int64)
+    kz2 = np.broadcast_to(kz * kz, m.shape).astype(np.int64)
+    if k_edges is None:
+        slot = np.floor(np.sqrt(m.astype(F64))).astype(np.int64) - 1
+        nb = math.isqrt(3 * (n // 2) ** 2)
+        slot = np.where(slot < nb, slot, -1)
+    else:
+        kedges = np.asarray(k_edges, F64) / (2.0 * np.pi / box_size)
+        nb = kedges.size - 1
+        slot = jnp_histogram_slots(np.sqrt(m.astype(F64)), kedges)
+    counts = np.bincount(slot[slot >= 0], minlength=nb)[:nb]
+    norm = (box_size / n ** 2) ** 3
+    g = np.asarray(grad_pk, F64).reshape(nb, 3)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        coef = np.where(counts[:, None] > 0, g * norm * np.array([1.0, 5.0, 9.0]) / counts[:, None], 0.0)
+    L2, L4 = legendre_weights(kz2, m)
+    s = np.where(slot >= 0, slot, 0)
+    G = window_correction(n, 2 * p) * (coef[s, 0] + coef[s, 1] * L2 + coef[s, 2] * L4)
+    G = np.where((slot >= 0) & (m > 0), G, 0.0)
+    A = 2.0 * G * dk
+    # dLoss/d delta(x) = Re sum_{half} A_k e^{+i k.x}
+    full = np.zeros((n, n, n), complex)
+    full[:, :, : n // 2 + 1] = A
+    gx = (np.fft.ifftn(full) * n ** 3).real.reshape(-1)
+    gbar = float(np.dot(gx, rho)) / M
+    ga = np.zeros((pos[0].size, 3))
+    gw = np.zeros(pos[0].size)
+    for flat, valid, W, dW in corners:
+        gv = np.where(valid, gx[np.where(valid, flat, 0)], 0.0)
+        vm = valid.astype(F64)
+        for ax in range(3):
+            ga[:, ax] += (gv - gbar) * dW[ax] * vm
+        gw += (gv - gbar) * W * vm
+    grad_pos = S * inv * w[:, None] * ga
+    grad_w = S * gw
+    return grad_pos, grad_w
