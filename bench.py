@@ -280,6 +280,105 @@ def bench_batched(dev, reps=5):
                     "inputs resident in HBM"}
 
 
+SERVE_REQUESTS = 2000
+SERVE_NS = (128, 256, 384, 512, 768, 1024)
+
+
+def make_serve_requests(dev, count=SERVE_REQUESTS, seed=5000):
+    """Requests of the serving benchmark: each draws n from {128, ..., 1024} and N_p
+    log-uniform in [1e5, 3e7]; its particles are the first N_p of one shuffled lognormal
+    catalogue (L = 1500 Mpc/h, 3e7 particles), i.e. a random subsample of the same field."""
+    pool = make_catalogue(512, BOX, 3.0e7 / BOX ** 3, dev, seed=seed)
+    reqs = []
+    kf = 2 * np.pi / BOX
+    for i in range(count):
+        rng = np.random.default_rng(seed + 1 + i)
+        n = int(rng.choice(SERVE_NS))
+        npart = min(int(np.exp(rng.uniform(np.log(1e5), np.log(3e7)))), pool.shape[0])
+        p = pool[:npart]
+        reqs.append(dict(x=p[:, 0], y=p[:, 1], z=p[:, 2], box_size=BOX, n_bins=n, mas="CIC_REF",
+                         k_edges=kf * (0.5 + np.arange(n // 2 + 1)), shot_noise=BOX ** 3 / npart))
+    return pool, reqs
+
+
+def bench_served(dev, plain_sample=300):
+    """Serving benchmark: sequential plain estimate() calls vs PkServer, at saturation and
+    under Poisson arrivals at 0.8x the plain path's measured capacity."""
+    import torch
+    from jax_powspec_amd import correlations
+    from jax_powspec_amd.serve import PkServer
+    log("served: generating requests ...")
+    pool, reqs = make_serve_requests(dev)
+    torch.cuda.synchronize()
+
+    def plain(r):
+        k, pk, nm = correlations.estimate(r["x"], r["y"], r["z"], box_size=r["box_size"], n_bins=r["n_bins"],
+                                          k_edges=r["k_edges"], mas=r["mas"], shot_noise=r["shot_noise"])
+        return pk.cpu().numpy()
+
+    for n in SERVE_NS:   # warm plans and kernels
+        plain(next(r for r in reqs if r["n_bins"] == n))
+    torch.cuda.synchronize()
+    sample = reqs[:plain_sample]
+    t0 = time.perf_counter()
+    lat_plain = []
+    for r in sample:
+        t = time.perf_counter()
+        plain(r)
+        lat_plain.append(time.perf_counter() - t)
+    plain_s = time.perf_counter() - t0
+    plain_rate = len(sample) / plain_s
+    log(f"served: plain {plain_rate:.1f} req/s on {len(sample)} requests")
+    srv = PkServer(device=dev, max_batch=8)
+    try:
+        for n in SERVE_NS:   # warm the engines (and their graphs) once
+            srv.submit(next(r for r in reqs if r["n_bins"] == n)).result(timeout=300)
+        srv.completed.clear()
+        futs = [srv.submit(r) for r in reqs]
+        for f in futs:
+            f.result(timeout=600)
+        sat = srv.stats()
+        log(f"served: saturation {sat['throughput_per_s']:.1f} req/s")
+        srv.completed.clear()
+        lam = 0.8 * plain_rate
+        rng = np.random.default_rng(6000)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
+        t0 = time.perf_counter()
+        futs = []
+        for r, ta in zip(reqs, arrivals):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(srv.submit(r))
+        for f in futs:
+            f.result(timeout=600)
+        poi = srv.stats()
+        log(f"served: poisson {poi['throughput_per_s']:.1f} req/s, p50 {poi['p50_ms']:.2f} ms, p99 {poi['p99_ms']:.2f} ms")
+        # the same requests' results alone (plain) and from the server agree
+        worst = 0.0
+        for f in futs[:40]:
+            want = plain(f.req)
+            got = f.result()[1]
+            ok = ~np.isnan(want)
+            scale = np.nanmax(np.abs(want), axis=0, keepdims=True)   # per multipole
+            worst = max(worst, float(np.max((np.abs(got - want) / scale)[ok])))
+    finally:
+        srv.close()
+    lp = np.array(lat_plain) * 1e3
+    return {"requests": len(reqs), "mesh_sides": list(SERVE_NS), "particles_range": [1e5, 3e7],
+            "plain_sequential_req_per_s": round(plain_rate, 2),
+            "plain_p50_ms": round(float(np.percentile(lp, 50)), 3), "plain_p99_ms": round(float(np.percentile(lp, 99)), 3),
+            "plain_sample": len(sample),
+            "served_saturation_req_per_s": round(sat["throughput_per_s"], 2),
+            "served_saturation_speedup": round(sat["throughput_per_s"] / plain_rate, 3),
+            "served_saturation_p50_ms": round(sat["p50_ms"], 3), "served_saturation_p99_ms": round(sat["p99_ms"], 3),
+            "poisson_rate_req_per_s": round(lam, 2), "poisson_throughput_req_per_s": round(poi["throughput_per_s"], 2),
+            "poisson_p50_ms": round(poi["p50_ms"], 3), "poisson_p99_ms": round(poi["p99_ms"], 3),
+            "max_diff_vs_plain_first40_rel_to_multipole_max": worst, "errors": sat["errors"] + poi["errors"],
+            "note": "latency = submit to P_l on the host; catalogues resident in HBM; plain = correlations.estimate() "
+                    "one request at a time (capacity measured on the first requests)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,6 +391,8 @@ def main():
                     help="run a 1 GiB device-to-device copy first (HBM counter calibration)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mock batched comparison")
     ap.add_argument("--only-batched", action="store_true", help="run only the batched comparison")
+    ap.add_argument("--no-served", action="store_true", help="skip the serving benchmark")
+    ap.add_argument("--only-served", action="store_true", help="run only the serving benchmark")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the calibrated HBM traffic summary of two rocprofv3 --pmc runs and exit")
     ap.add_argument("--pmc-command", default="", help="command line recorded in the --pmc-summary header")
@@ -321,6 +422,9 @@ def main():
         del src, dst
     if args.only_batched:
         print(json.dumps({"batched": bench_batched(dev)}), flush=True)
+        return
+    if args.only_served:
+        print(json.dumps({"served": bench_served(dev)}), flush=True)
         return
     log("generating catalogue ...")
     pos = make_catalogue(N_MESH, BOX, NBAR, dev)
@@ -387,10 +491,14 @@ def main():
         roof_path = path_bytes / (ms * 1e-3) / 1e9
         log(f"whole-path algorithmic bytes {path_bytes / 1e9:.3f} GB -> {roof_path:.0f} GB/s "
             f"({roof_path / HBM_PEAK_GBS:.3f} of peak)")
-    batched = None
+    batched = served = None
+    del est
     if not args.no_batched:
-        del est
         batched = bench_batched(dev)
+        torch.cuda.empty_cache()
+    if not args.no_served:
+        served = bench_served(dev)
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -413,6 +521,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -198,6 +198,9 @@ typedef struct pk_job {
 typedef struct pk_batch pk_batch;
 
 #define PK_BATCH_GRAPH 1
+/* size the sort launches for the batch capacity (max_np, TSC) instead of the current jobs:
+ * fewer distinct launch shapes, so graph replays repeat when job sizes vary */
+#define PK_BATCH_CAPACITY_GRIDS 2
 
 int pk_batch_workspace_size(pk_plan* plan, int max_jobs, int64_t max_np, int max_nb, const int* tile,
                             size_t* bytes);

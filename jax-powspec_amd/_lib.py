@@ -34,6 +34,7 @@ _d = C.c_double
 _sz = C.c_size_t
 _psz = C.POINTER(C.c_size_t)
 BATCH_GRAPH = 1
+BATCH_CAPACITY_GRIDS = 2
 
 
 class PkJob(C.Structure):

@@ -87,7 +87,7 @@ class BatchEstimator:
     """
 
     def __init__(self, n: int, max_jobs: int, max_particles: int, max_bins: int, *, tile=None, graph: bool = True,
-                 device=None):
+                 device=None, capacity_grids: bool = False):
         self.device = _device.require_gpu(device)
         self.n = int(n)
         self.max_jobs = int(max_jobs)
@@ -105,7 +105,8 @@ class BatchEstimator:
             h = C.c_void_p()
             _lib.check("pk_batch_create", lib.pk_batch_create(
                 self.plan.handle, self.max_jobs, self.max_particles, self.max_bins, self._tile_ptr,
-                _lib.BATCH_GRAPH if graph else 0, _device.ptr(self.ws), self.ws_bytes, C.byref(h)))
+                (_lib.BATCH_GRAPH if graph else 0) | (_lib.BATCH_CAPACITY_GRIDS if capacity_grids else 0),
+                _device.ptr(self.ws), self.ws_bytes, C.byref(h)))
         self.handle = h
         t = (C.c_int * 3)()
         _lib.check("pk_batch_tile", lib.pk_batch_tile(self.handle, C.cast(t, C.c_void_p)))

@@ -731,6 +731,10 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     if (j.g.nkeys > d.max_nkeys) d.max_nkeys = j.g.nkeys;
     if (q.nb > d.max_nb) d.max_nb = q.nb;
   }
+  if (b->flags & PK_BATCH_CAPACITY_GRIDS) {
+    d.max_nblk = b->cap.nblk;
+    d.max_nkeys = b->cap.nkeys;
+  }
   hipStream_t s = as_stream(stream);
   if (njobs > 0) {
     e = check_hip(hipMemcpyAsync(b->d_jobs, h, (size_t)njobs * sizeof(Job), hipMemcpyHostToDevice, s),
