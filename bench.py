@@ -175,6 +175,44 @@ def cpu_baseline(pos_host, n, box, edges, shot):
                       f"({dt:.1f} s; FFT on {cores} threads, scatter and binning single-threaded NumPy)"}
 
 
+def bench_small(dev, steps=50, cpu=True):
+    """Plain small case: 1e6 uniform particles (seed 1001) in L = 1000 Mpc/h, 128^3 CIC
+    (reference stencil), monopole in k_F bins from k_F to k_N, shot noise 1/nbar."""
+    import torch
+    from jax_powspec_amd.correlations import Estimator
+    n, box, npart = 128, 1000.0, 1_000_000
+    pos_h = (np.random.default_rng(1001).random((npart, 3), dtype=np.float32) * np.float32(box)).astype(np.float32)
+    pos = torch.as_tensor(pos_h, device=dev)
+    kf = 2 * np.pi / box
+    edges = kf * np.arange(1, n // 2 + 1)
+    shot = box ** 3 / npart
+    est = Estimator(box_size=box, n_bins=n, max_particles=npart, k_edges=edges, mas="CIC_REF", shot_noise=shot,
+                    device=dev)
+    for _ in range(3):
+        est.run(pos[:, 0], pos[:, 1], pos[:, 2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        est.run(pos[:, 0], pos[:, 1], pos[:, 2])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": "1e6 uniform particles, L=1000 Mpc/h, 128^3 CIC (reference stencil), P_0 in 63 k_F bins",
+           "estimates_per_s": round(1e3 / ms, 1), "ms_per_estimate": round(ms, 4)}
+    if cpu:
+        from oracle import cpu_fp32
+        cores = cpu_fp32.ncores()
+        cpu_fp32.estimate_f32(pos_h[:, 0], pos_h[:, 1], pos_h[:, 2], box, n, edges, shot, workers=cores)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            cpu_fp32.estimate_f32(pos_h[:, 0], pos_h[:, 1], pos_h[:, 2], box, n, edges, shot, workers=cores)
+        dt = (time.perf_counter() - t0) / reps
+        out["cpu_estimates_per_s"] = round(1.0 / dt, 3)
+        out["cpu_cores"] = cores
+        out["gpu_over_cpu"] = round(dt * 1e3 / ms, 1)
+    return out
+
+
 BATCH_MOCKS = 256
 BATCH_NS = (64, 96, 128, 192, 256)
 
@@ -493,6 +531,8 @@ def main():
             f"({roof_path / HBM_PEAK_GBS:.3f} of peak)")
     batched = served = None
     del est
+    small = bench_small(dev, cpu=not args.no_cpu)
+    log(f"small: {small}")
     if not args.no_batched:
         batched = bench_batched(dev)
         torch.cuda.empty_cache()
@@ -520,6 +560,7 @@ def main():
                                "P_0,2,4 in 256 k-bins", "mesh": N_MESH, "particles": npart, "bins": NBINS},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "small": small,
         "batched": batched,
         "served": served,
     }
