@@ -197,8 +197,6 @@ __device__ __forceinline__ float dpp_move(float v) {
 constexpr int DPP_ROW_SHR = 0x110;   // + shift (1..15), within rows of 16 lanes
 constexpr int DPP_WAVE_SHR1 = 0x138; // lane l reads lane l - 1
 constexpr int DPP_WAVE_SHL1 = 0x130; // lane l reads lane l + 1
-constexpr int DPP_BCAST15 = 0x142;   // a row reads lane 15 of the row before (rows 1, 3 here)
-constexpr int DPP_BCAST31 = 0x143;   // a row reads lane 31 (rows 2, 3)
 
 // one step of a segmented inclusive scan: (v, f) <- (v + v_src, f_src) where this lane's
 // run has not started yet (f == 0); a lane without a source reads (0, 0) and is unchanged
@@ -218,12 +216,13 @@ __device__ __forceinline__ void seg_step(float& a, float& b, float& c, float& d,
   }
 }
 
-// Sums of the values over each run of lanes with equal `slot` (lane order), left in the
-// run's last lane; returns true in that lane.  A segmented inclusive scan over the 64
-// lanes: shifts 1, 2, 4, 8 inside rows of 16, then row 0's total into row 1 and row 2's
-// into row 3 (bcast15), then rows 0-1's total into rows 2 and 3 (bcast31).
-// Float partial sums of at most 64 terms (depth-6 tree: relative error <= 6 ulp of the
-// run's absolute sum).  Must be called by all 64 lanes.
+// Sums of the values over each run of lanes with equal `slot` inside a row of 16 lanes
+// (lane order), left in the run's last lane of the row; returns true in that lane.  A
+// segmented inclusive scan with shifts 1, 2, 4, 8 inside rows of 16: a run that crosses a
+// row boundary is flushed once per row, which costs an extra histogram update but saves
+// the two cross-row steps (row_bcast15 into rows 1 and 3, then row_bcast31 into rows 2
+// and 3) of a full-wave scan.  Float partial sums of at most 16 terms (depth-4 tree:
+// relative error <= 4 ulp of the run's absolute sum).  Must be called by all 64 lanes.
 __device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, float& c, float& d, float& e, bool with_e) {
   const int lane = threadIdx.x & 63;
   // every cross-lane move runs with all 64 lanes active (a lane the exec mask leaves out
@@ -231,13 +230,11 @@ __device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, floa
   const int prev = dpp_move<DPP_WAVE_SHR1>(slot + 1) - 1;   // lane 0 reads -1 (no source)
   int f = (lane == 0 || prev != slot) ? 1 : 0;               // this lane starts a run
   const int next_starts = dpp_move<DPP_WAVE_SHL1>(f);
-  const bool tail = (lane == 63) | (next_starts != 0);
+  const bool tail = ((lane & 15) == 15) | (next_starts != 0);
   seg_step<DPP_ROW_SHR + 1, 0xf>(a, b, c, d, e, f, with_e);
   seg_step<DPP_ROW_SHR + 2, 0xf>(a, b, c, d, e, f, with_e);
   seg_step<DPP_ROW_SHR + 4, 0xf>(a, b, c, d, e, f, with_e);
   seg_step<DPP_ROW_SHR + 8, 0xf>(a, b, c, d, e, f, with_e);
-  seg_step<DPP_BCAST15, 0xa>(a, b, c, d, e, f, with_e);
-  seg_step<DPP_BCAST31, 0xc>(a, b, c, d, e, f, with_e);
   return tail;
 }
 
@@ -254,8 +251,9 @@ constexpr int NACC = 5;       // S0, S2, S4, count, sum sqrt(m)
 template <int N> struct XBinLds {
   static constexpr size_t TILE = (size_t)N * (FFTCfg<N>::CX + 1) * 8;
   static constexpr size_t HIST = (size_t)(HB_LDS + 1) * 4 + 4 * HB_LDS * 8 + HB_LDS * 4;
+  // LDS-limited waves per SIMD (the histogram in HBM: no limit is imposed)
   template <bool LH> static constexpr int wpe() {
-    return cmax(1, cmin(4, (int)((160 * 1024) / (TILE + (LH ? HIST : 64))) * (FFTCfg<N>::NTX / 64) / 4));
+    return LH ? cmax(1, cmin(4, (int)((160 * 1024) / (TILE + HIST)) * (FFTCfg<N>::NTX / 64) / 4)) : 1;
   }
 };
 
@@ -298,54 +296,92 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   __syncthreads();
   const int* thr = LH ? thr_s : thr_g;
   const bool want_k = (flags & PK_FLAG_KMEAN) != 0;
+  // pair tasks: |kx| = q in [0, N/2], column c; entries q and N-q share m and mu.  Every
+  // thread keeps one column (NT is a multiple of C) and walks q in steps of NT / C, so the
+  // loop is wave-uniform (idle lanes carry slot -1) and the wave can combine lanes that
+  // share a bin before touching the histogram.
+  static_assert(NT % C == 0, "binning threads keep one column");
+  constexpr int QSTEP = NT / C;
+  constexpr int NIT = (H + 1 + QSTEP - 1) / QSTEP;
+  const int c = tid % C, q0 = tid / C;
+  // N = 64, 128: the next tile's columns are loaded into registers while this one is
+  // transformed and binned (for other N the registers would spill or cost a block)
+  constexpr bool XPF = LH && (N == 64 || N == 128);
+  constexpr int XPER = (N * C + NT - 1) / NT;
+  float2 xpre[XPF ? XPER : 1];
+  auto load_tile = [&](int t) {
+    const float2* src = spec + (size_t)(t / YT) * N * N + (size_t)(t % YT) * C;
+#pragma unroll
+    for (int k = 0; k < XPER; ++k) {
+      const int i = tid + k * NT;
+      if (i < N * C) xpre[k] = src[(size_t)(i / C) * N + (i % C)];
+    }
+  };
+  if constexpr (XPF) {
+    if (bx < NTILES) load_tile(bx);
+  }
   for (int tile = bx; tile < NTILES; tile += nbx) {
     const int kz = tile / YT, yt = tile % YT;
-    const float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
-    for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
-    __syncthreads();
-    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
-    const float wz = wt[kz];
     const int kz2 = kz * kz;
-    // pair tasks: |kx| = q in [0, N/2], column c; entries q and N-q share m and mu.  The
-    // loop is wave-uniform (idle lanes carry slot -1) so the wave can combine lanes that
-    // share a bin before touching the histogram.
-    for (int i0 = 0; i0 < (H + 1) * C; i0 += NT) {
-      const int i = i0 + tid;
-      int slot = -1;
+    const int y = yt * C + c;
+    const int ky = y <= H ? y : y - N;
+    const int r2 = ky * ky + kz2;
+    // bins and window factors do not depend on the transform: the first iteration's table
+    // reads are issued before the tile's load and FFT, each later one an iteration ahead
+    int slot_next = q0 <= H ? find_bin(q0 * q0 + r2, thr, jst, nb) : -1;
+    float wq_next = q0 <= H ? wt[q0] : 0.0f;
+    const float wyz = wt[y] * wt[kz];
+    if constexpr (XPF) {
+#pragma unroll
+      for (int k = 0; k < XPER; ++k) {
+        const int i = tid + k * NT;
+        if (i < N * C) buf[(i / C) * PS + (i % C)] = xpre[k];
+      }
+    } else {
+      const float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
+      for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
+    }
+    __syncthreads();
+    if constexpr (XPF) {
+      if (tile + nbx < NTILES) load_tile(tile + nbx);
+    }
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+#pragma unroll 1
+    for (int it = 0; it < NIT; ++it) {
+      const int q = q0 + it * QSTEP;
+      const int slot = slot_next;
+      const float wqi = wq_next;
+      const int qn = q + QSTEP;
+      slot_next = qn <= H ? find_bin(qn * qn + r2, thr, jst, nb) : -1;
+      wq_next = qn <= H ? wt[qn] : 0.0f;
       float p2 = 0.0f, v2 = 0.0f, v4 = 0.0f, vc = 0.0f, vk = 0.0f;
-      if (i < (H + 1) * C) {
-        const int c = i % C, q = i / C;
-        const int y = yt * C + c;
-        const int ky = y <= H ? y : y - N;
-        const int m = q * q + ky * ky + kz2;
-        slot = find_bin(m, thr, jst, nb);
-        if (slot >= 0) {
-          const float2 a = buf[q * PS + c];
-          p2 = a.x * a.x + a.y * a.y;
-          int cnt = 1;
-          if (q != 0 && q != H) {
-            const float2 bb = buf[(N - q) * PS + c];
-            p2 += bb.x * bb.x + bb.y * bb.y;
-            cnt = 2;
-          }
-          p2 *= wt[q] * wt[y] * wz;
-          // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
-          //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
-          // so L2 is exactly 0 where 3 kz^2 = m.  The k = 0 mode's power is added by the
-          // finalize from the FP64 sum of the mesh (mu = 0 there, :37).
-          if (m > 0) {
-            const float rm = 1.0f / (float)m;
-            const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
-            const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
-            const float l4 = 0.125f * (float)num4 * rm * rm;
-            v2 = p2 * l2;
-            v4 = p2 * l4;
-          } else {
-            p2 = 0.0f;
-          }
-          vc = (float)cnt;
-          vk = want_k ? (float)cnt * sqrtf((float)m) : 0.0f;
+      if (slot >= 0) {
+        const int m = q * q + r2;
+        const float2 a = buf[q * PS + c];
+        p2 = a.x * a.x + a.y * a.y;
+        int cnt = 1;
+        if (q != 0 && q != H) {
+          const float2 bb = buf[(N - q) * PS + c];
+          p2 += bb.x * bb.x + bb.y * bb.y;
+          cnt = 2;
         }
+        p2 *= wqi * wyz;
+        // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
+        //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
+        // so L2 is exactly 0 where 3 kz^2 = m.  The k = 0 mode's power is added by the
+        // finalize from the FP64 sum of the mesh (mu = 0 there, :37).
+        if (m > 0) {
+          const float rm = 1.0f / (float)m;
+          const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
+          const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
+          const float l4 = 0.125f * (float)num4 * rm * rm;
+          v2 = p2 * l2;
+          v4 = p2 * l4;
+        } else {
+          p2 = 0.0f;
+        }
+        vc = (float)cnt;
+        vk = want_k ? (float)cnt * sqrtf((float)m) : 0.0f;
       }
       // runs of equal slot in lane order -> one histogram update per run (its last lane)
       const bool tail = wave_run_sums(slot, p2, v2, v4, vc, vk, want_k);

@@ -1,12 +1,13 @@
 """Host model of the x pass's wave-level run sums (wave_run_sums in csrc/pk_fft.hip).
 
-The kernel combines the 64 lanes of a wave that share a bin before touching the LDS
-histogram: a segmented inclusive scan built from DPP lane moves.  This test replays the
-exact step sequence (row_shr 1, 2, 4, 8 inside rows of 16; row_bcast15 into rows 1 and 3;
-row_bcast31 into rows 2 and 3; lanes without a source read 0) on random slot patterns and
-checks that every run's last lane holds the run's sum and that summing the tails equals a
-per-bin brute-force histogram.  A wrong row mask (e.g. bcast15 into row 2 as well) counts a
-row twice and fails here.
+The kernel combines lanes of a wave that share a bin before touching the LDS histogram:
+a segmented inclusive scan built from DPP lane moves, row_shr 1, 2, 4, 8 inside rows of 16
+lanes (lanes without a source read 0), with the run flags from wave_shr / wave_shl moves.
+This test replays that exact sequence on random slot patterns and checks that the last
+lane of every run inside a row holds the run's sum and that summing those lanes equals a
+per-bin brute-force histogram.  The full-wave variant (row_bcast15 into rows 1 and 3, then
+row_bcast31 into rows 2 and 3) is checked the same way, and a wrong bcast15 row mask is
+shown to count a row twice.
 """
 import numpy as np
 import pytest
@@ -37,15 +38,19 @@ def dpp(v, ctrl, rows=0xF):
     return out
 
 
-def wave_run_sums(slot, vals):
+ROW_STEPS = ((("row_shr", 1), 0xF), (("row_shr", 2), 0xF), (("row_shr", 4), 0xF), (("row_shr", 8), 0xF))
+WAVE_STEPS = ROW_STEPS + (("bcast15", 0xA), ("bcast31", 0xC))
+
+
+def wave_run_sums(slot, vals, rows_only=True):
     """Replays the kernel's sequence; returns (tail mask, scanned values)."""
     lane = np.arange(W)
     prev = dpp(slot + 1, "wave_shr1") - 1
     f = ((lane == 0) | (prev != slot)).astype(np.int64)
-    tail = (lane == W - 1) | (dpp(f, "wave_shl1") != 0)
+    nxt = dpp(f, "wave_shl1") != 0
+    tail = (((lane & 15) == 15) if rows_only else (lane == W - 1)) | nxt
     v = vals.astype(np.float64).copy()
-    for ctrl, rows in ((("row_shr", 1), 0xF), (("row_shr", 2), 0xF), (("row_shr", 4), 0xF), (("row_shr", 8), 0xF),
-                       ("bcast15", 0xA), ("bcast31", 0xC)):
+    for ctrl, rows in (ROW_STEPS if rows_only else WAVE_STEPS):
         pv = np.stack([dpp(v[:, k], ctrl, rows) for k in range(v.shape[1])], axis=1)
         pf = dpp(f, ctrl, rows)
         upd = f == 0
@@ -54,8 +59,9 @@ def wave_run_sums(slot, vals):
     return tail, v
 
 
+@pytest.mark.parametrize("rows_only", [True, False])
 @pytest.mark.parametrize("pattern", ["random", "runs", "single", "mostly_invalid", "alternating"])
-def test_run_sums_match_brute_force(pattern):
+def test_run_sums_match_brute_force(pattern, rows_only):
     rng = np.random.default_rng(hash(pattern) % 2**32)
     for _ in range(50):
         if pattern == "random":
@@ -71,10 +77,12 @@ def test_run_sums_match_brute_force(pattern):
         else:
             slot = np.arange(W) % 2
         vals = rng.random((W, 3))
-        tail, v = wave_run_sums(slot, vals)
-        # every run (maximal stretch of equal slot in lane order) ends in exactly one tail
-        starts = np.r_[True, slot[1:] != slot[:-1]]
-        ends = np.r_[slot[1:] != slot[:-1], True]
+        tail, v = wave_run_sums(slot, vals, rows_only)
+        # every run (maximal stretch of equal slot in lane order, cut at row ends when the
+        # scan stays inside rows) ends in exactly one tail
+        cut = (np.arange(W) % 16 == 0) if rows_only else (np.arange(W) == 0)
+        starts = np.r_[True, slot[1:] != slot[:-1]] | cut
+        ends = np.r_[starts[1:], True]
         np.testing.assert_array_equal(tail, ends)
         run_id = np.cumsum(starts) - 1
         for r in np.unique(run_id):
@@ -105,5 +113,5 @@ def test_wrong_row_mask_would_double_count():
         v[upd] += pv[upd]
         f = np.where(upd, pf, f)
     assert v[47, 0] != 48   # the mask 0xe adds row 1 twice into row 2
-    tail, good = wave_run_sums(slot, vals)
+    tail, good = wave_run_sums(slot, vals, rows_only=False)
     assert good[47, 0] == 48 and good[63, 0] == 16
