@@ -36,95 +36,163 @@ template <int N, int TY_> struct FusedCfg {
   static constexpr int PER = (TY * N + NT - 1) / NT;     // cells per thread in the conversion
 };
 
+// Bucket descriptors of one tile in LDS (see gather_buckets): keys, first record and
+// exclusive prefix of sizes.  Block-cooperative; ends with a barrier.
+struct TileBuckets {
+  int key[MAX_TILE_BUCKETS];
+  int base[MAX_TILE_BUCKETS];
+  int pre[MAX_TILE_BUCKETS + 1];
+  int nbk;
+};
+
 template <int N, int TY>
-__device__ __forceinline__ void fused_body(const Job& j, int t, const float2* __restrict__ twg) {
+__device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const float2* __restrict__ twg) {
   using Cf = FusedCfg<N, TY>;
   constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
+  constexpr int U = NT >= 1024 ? 2 : 4;   // records per thread prefetched for the next tile
   extern __shared__ float2 buf[];   // FFT rows: C rows of CS complex = 2 * CS floats per row
   __shared__ float2 tw[N];
-  __shared__ int s_key[MAX_TILE_BUCKETS], s_base[MAX_TILE_BUCKETS], s_pre[MAX_TILE_BUCKETS + 1];
-  __shared__ int s_nbk;
+  __shared__ TileBuckets tb[2];
   __shared__ double red[NT];
   const ScatterArgs a = j.a;
   const TileGeom g = j.g;
   const void* sorted = j.sorted;
+  const bool has_w = a.w != nullptr;
+  const bool tsc = a.stencil == PK_STENCIL_TSC;
+  const bool irregular = a.stencil == PK_STENCIL_CIC_REF;
   float* rows = reinterpret_cast<float*>(buf);
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(buf);   // [C][N] when FIXED
   const int tid = threadIdx.x;
-  const int xt = t / g.nyt, yt = t % g.nyt;
-  const int y0 = yt * C;
+  const int ntiles = g.ntiles;
+  if (bx >= ntiles) return;   // whole block
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
-  if constexpr (Cf::FIXED) {
-    for (int i = tid; i < C * N; i += NT) acc[i] = 0ull;
-  } else {
-    for (int i = tid; i < C * 2 * CS; i += NT) rows[i] = 0.0f;
-  }
-  gather_buckets(g, N, a.stencil == PK_STENCIL_TSC, a.stencil == PK_STENCIL_CIC_REF, xt, yt, j.key_start, s_key,
-                 s_base, s_pre, &s_nbk);
   const float scale = (float)j.aux[0];
-  deposit_buckets(a, g, sorted, s_base, s_pre, s_nbk, NT, xt, y0, [&](int ly, int iz, float wt) {
-    if constexpr (Cf::FIXED) atomicAdd(&acc[ly * N + iz], to_fixed(wt, scale));
-    else atomicAdd(&rows[ly * 2 * CS + iz], wt);
-  });
-  __syncthreads();
-  // tile mass (FP64) and rho - c into the FFT rows
+  const double inv_scale = j.aux[1];
   const float c0 = (float)j.stats[3];
-  double msum = 0.0;
-  if constexpr (Cf::FIXED) {
-    const double inv = j.aux[1];
-    float v[Cf::PER];
+  float2* spec = j.spec;
+  // record idx (< total) of the tile buckets B; a thread visits increasing idx, so its
+  // bucket cursor b only moves forward
+  auto record = [&](const TileBuckets& B, int idx, int& b) -> float4 {
+    while (b + 1 < B.nbk && B.pre[b + 1] <= idx) ++b;
+    return load_record(sorted, (size_t)B.base[b] + (idx - B.pre[b]), has_w);
+  };
+  // Software pipeline over this block's tiles: the next tile's bucket descriptors are set
+  // up after the current deposit, and its first U * NT records are loaded into registers
+  // before the current tile's FFT, so their latency overlaps the transform and the writes.
+  gather_buckets(g, N, tsc, irregular, bx / g.nyt, bx % g.nyt, j.key_start, tb[0].key, tb[0].base, tb[0].pre,
+                 &tb[0].nbk);
+  float4 rp[U];
+  auto prefetch = [&](const TileBuckets& B) {
+    const int total = B.pre[B.nbk];
+    int b = 0;
 #pragma unroll
-    for (int k = 0; k < Cf::PER; ++k) {
-      const int i = tid + k * NT;
-      if (i < C * N) {
-        const double q = (double)(long long)acc[i] * inv;
-        msum += q;
-        v[k] = (float)q - c0;
+    for (int u = 0; u < U; ++u) {
+      const int idx = tid + u * NT;
+      if (idx < total) rp[u] = record(B, idx, b);
+    }
+  };
+  prefetch(tb[0]);
+  int cur = 0;
+  for (int t = bx; t < ntiles; t += nbx) {
+    const int xt = t / g.nyt, yt = t % g.nyt;
+    const int y0 = yt * C;
+    const TileBuckets& B = tb[cur];
+    if constexpr (Cf::FIXED) {
+      for (int i = tid; i < C * N; i += NT) acc[i] = 0ull;
+    } else {
+      for (int i = tid; i < C * 2 * CS; i += NT) rows[i] = 0.0f;
+    }
+    __syncthreads();
+    auto dep = [&](const float4& r) {
+      for_each_corner_on_x(a, r.x, r.y, r.z, r.w, xt, [&](int iy, int iz, float wt) {
+        const int ly = iy - y0;
+        if ((unsigned)ly < (unsigned)C) {
+          if constexpr (Cf::FIXED) atomicAdd(&acc[ly * N + iz], to_fixed(wt, scale));
+          else atomicAdd(&rows[ly * 2 * CS + iz], wt);
+        }
+      });
+    };
+    const int total = B.pre[B.nbk];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (tid + u * NT < total) dep(rp[u]);
+    int bc = 0;
+    for (int idx0 = tid + U * NT; idx0 < total; idx0 += U * NT) {
+      float4 r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = idx0 + u * NT;
+        if (idx < total) r[u] = record(B, idx, bc);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (idx0 + u * NT < total) dep(r[u]);
+    }
+    const int tn = t + nbx;
+    if (tn < ntiles)   // block-uniform; its barriers also close the deposit
+      gather_buckets(g, N, tsc, irregular, tn / g.nyt, tn % g.nyt, j.key_start, tb[cur ^ 1].key, tb[cur ^ 1].base,
+                     tb[cur ^ 1].pre, &tb[cur ^ 1].nbk);
+    else
+      __syncthreads();
+    // tile mass (FP64) and rho - c into the FFT rows
+    double msum = 0.0;
+    if constexpr (Cf::FIXED) {
+      float v[Cf::PER];
+#pragma unroll
+      for (int k = 0; k < Cf::PER; ++k) {
+        const int i = tid + k * NT;
+        if (i < C * N) {
+          const double q = (double)(long long)acc[i] * inv_scale;
+          msum += q;
+          v[k] = (float)q - c0;
+        }
+      }
+      __syncthreads();   // every cell read before the float rows overwrite them
+#pragma unroll
+      for (int k = 0; k < Cf::PER; ++k) {
+        const int i = tid + k * NT;
+        if (i < C * N) rows[(i / N) * 2 * CS + (i % N)] = v[k];
+      }
+    } else {
+      for (int i = tid; i < C * N; i += NT) {
+        const int r = i / N, z = i - r * N;
+        float* p = &rows[r * 2 * CS + z];
+        const float v = *p;
+        msum += (double)v;
+        *p = v - c0;
       }
     }
-    __syncthreads();   // every cell read before the float rows overwrite them
-#pragma unroll
-    for (int k = 0; k < Cf::PER; ++k) {
-      const int i = tid + k * NT;
-      if (i < C * N) rows[(i / N) * 2 * CS + (i % N)] = v[k];
+    const double m = block_sum_f64(msum, red, NT);   // ends with a barrier
+    if (tid == 0) j.tile_mass[t] = m;
+    if (tn < ntiles) prefetch(tb[cur ^ 1]);
+    // z transform of the C rows (pairs of reals as NC complex, then the split)
+    lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+    for (int i = tid; i < (NC + 1) * C; i += NT) {
+      const int c = i % C, k = i / C;
+      const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
+      const float2 zr = buf[c * CS + (k == 0 ? 0 : NC - k)];
+      const float2 zc = make_float2(zr.x, -zr.y);
+      const float2 fe = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+      const float2 dd = csub(zk, zc);
+      const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
+      spec[((size_t)k * N + xt) * N + (size_t)y0 + c] = cadd(fe, cmul(tw[k], fo));
     }
-  } else {
-    for (int i = tid; i < C * N; i += NT) {
-      const int r = i / N, z = i - r * N;
-      float* p = &rows[r * 2 * CS + z];
-      const float v = *p;
-      msum += (double)v;
-      *p = v - c0;
-    }
-  }
-  const double m = block_sum_f64(msum, red, NT);   // ends with a barrier
-  if (tid == 0) j.tile_mass[t] = m;
-  // z transform of the C rows (pairs of reals as NC complex, then the split)
-  lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
-  float2* spec = j.spec;
-  for (int i = tid; i < (NC + 1) * C; i += NT) {
-    const int c = i % C, k = i / C;
-    const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
-    const float2 zr = buf[c * CS + (k == 0 ? 0 : NC - k)];
-    const float2 zc = make_float2(zr.x, -zr.y);
-    const float2 fe = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
-    const float2 dd = csub(zk, zc);
-    const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
-    spec[((size_t)k * N + xt) * N + (size_t)y0 + c] = cadd(fe, cmul(tw[k], fo));
+    __syncthreads();   // the split has read the rows before the next tile clears them
+    cur ^= 1;
   }
 }
 
 template <int N, int TY>
 __global__ void __launch_bounds__((FusedCfg<N, TY>::NT)) k_fused_deposit_zpass(Job j, const float2* __restrict__ tw) {
-  fused_body<N, TY>(j, blockIdx.x, tw);
+  fused_body<N, TY>(j, blockIdx.x, gridDim.x, tw);
 }
 
 template <int N, int TY>
 __global__ void __launch_bounds__((FusedCfg<N, TY>::NT))
 k_fused_deposit_zpass_jobs(const Job* __restrict__ jobs, const float2* __restrict__ tw) {
   const Job& j = jobs[blockIdx.y];
-  if (!j.active || (int)blockIdx.x >= j.g.ntiles) return;
-  fused_body<N, TY>(j, blockIdx.x, tw);
+  if (!j.active) return;
+  fused_body<N, TY>(j, blockIdx.x, gridDim.x, tw);
 }
 
 template <int N, int TY> struct FusedLaunch {
@@ -139,6 +207,20 @@ template <int N, int TY> struct FusedLaunch {
                                          (int)Cf::LDS),
                      "LDS attribute");
   }
+  // persistent grid: resident blocks of the whole GPU (LDS-limited), at most one per tile
+  static int grid() {
+    static int g = 0;
+    if (!g) {
+      int dev = 0, occ = 0;
+      (void)hipGetDevice(&dev);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fused_deposit_zpass<N, TY>, Cf::NT, Cf::LDS) !=
+              hipSuccess ||
+          occ < 1)
+        occ = 1;
+      g = num_cus(dev) * occ;
+    }
+    return g;
+  }
   static int run(const Job& j, const float2* tw, hipStream_t s) {
     static bool done = false;
     if (!done) {
@@ -146,8 +228,9 @@ template <int N, int TY> struct FusedLaunch {
       if (e) return e;
       done = true;
     }
+    const int gx = j.g.ntiles < grid() ? j.g.ntiles : grid();
     StageTimer t(ST_DEPOSIT, s);
-    hipLaunchKernelGGL((k_fused_deposit_zpass<N, TY>), dim3(j.g.ntiles), dim3(Cf::NT), Cf::LDS, s, j, tw);
+    hipLaunchKernelGGL((k_fused_deposit_zpass<N, TY>), dim3(gx), dim3(Cf::NT), Cf::LDS, s, j, tw);
     return check_hip(hipGetLastError(), "fused deposit + z pass launch");
   }
   static int run_jobs(const Job* jobs, int njobs, int ntiles, const float2* tw, hipStream_t s) {
@@ -157,8 +240,11 @@ template <int N, int TY> struct FusedLaunch {
       if (e) return e;
       done = true;
     }
+    // one block per tile: jobs of very different sizes share the launch, and dynamic block
+    // scheduling balances them better than a fixed share of a persistent grid per job
+    const int gx = ntiles;
     StageTimer t(ST_DEPOSIT, s);
-    hipLaunchKernelGGL((k_fused_deposit_zpass_jobs<N, TY>), dim3(ntiles, njobs), dim3(Cf::NT), Cf::LDS, s, jobs, tw);
+    hipLaunchKernelGGL((k_fused_deposit_zpass_jobs<N, TY>), dim3(gx, njobs), dim3(Cf::NT), Cf::LDS, s, jobs, tw);
     return check_hip(hipGetLastError(), "batched fused deposit + z pass launch");
   }
 };
