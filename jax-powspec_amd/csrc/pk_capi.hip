@@ -152,6 +152,7 @@ int make_args(ScatterArgs* a, const float* x, int64_t sx, const float* y, int64_
   a->n = n;
   a->stencil = stencil;
   a->wrap_mode = wrap_mode;
+  a->aos = (sx == 3 && sy == 3 && sz == 3 && y == x + 1 && z == x + 2) ? 1 : 0;
   return PK_OK;
 }
 
@@ -705,6 +706,13 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
   Job* h = b->h_jobs[k];
   BatchDims d{njobs, 1, 1, 1};
   const int tile[3] = {1, b->cap.ty, 0};
+  // counting chunk of a batch: about four rounds of blocks over the whole GPU for all its
+  // particles (jobs of very different sizes balance), never more blocks per job than its slot holds
+  int64_t total_np = 0;
+  for (int i = 0; i < njobs; ++i) total_np += jobs[i].np > 0 ? jobs[i].np : 0;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int64_t batch_chunk = (total_np + 4LL * num_cus(dev) - 1) / (4LL * num_cus(dev));
   for (int i = 0; i < njobs; ++i) {
     const pk_job& q = jobs[i];
     Job j = b->slot[i];
@@ -719,6 +727,13 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     if (!e) e = make_tile_geom(b->n, q.np, q.stencil, tile, &j.g, &direct);
     if (!e && (direct || !fused_supported(b->n, j.g))) e = set_error(PK_E_UNSUPPORTED, "job %d: no fused path", i);
     if (e) return e;
+    {
+      int64_t c = batch_chunk;
+      const int64_t floor_c = (q.np + b->cap.nblk - 1) / b->cap.nblk;   // at most cap.nblk blocks
+      if (c < floor_c) c = floor_c;
+      set_chunk(&j.g, q.np, c);
+      if (j.g.nblk > b->cap.nblk) return set_error(PK_E_INVALID, "job %d: counting blocks exceed the slot", i);
+    }
     j.active = 1;
     if (q.stats) j.stats = q.stats;
     set_bin_fields(j, b->plan, j.spec, q.thr, q.jstart, q.nb, q.window_p, q.flags | PK_FLAG_ZERO_DC, q.box_size,

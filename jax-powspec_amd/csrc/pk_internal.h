@@ -20,6 +20,7 @@ struct ScatterArgs {
   int n;
   int stencil;
   int wrap_mode;
+  int aos;      // x, y, z interleaved (y = x + 1, z = x + 2, strides 3): one 12-byte load
 };
 
 // Row-bundle tiles of the sorted deposit: tile (xt, yt) = cells x = xt,
@@ -30,7 +31,7 @@ struct TileGeom {
   int ntiles;   // n * nyt
   int ncls;     // y classes: 2 (CIC variants) or 3 (TSC)
   int nkeys;    // ntiles * ncls + 1 (last key = irregular particles)
-  int chunk;    // particles per counting block (<= 65535: 16-bit LDS counters)
+  int chunk;    // particles per counting block (<= 65535: packed 16-bit LDS counters)
   int nblk;     // counting blocks
   int fixed;    // 1: deposit into 64-bit fixed-point LDS cells (ty * n * 8 bytes fit), 0: float
 };
@@ -104,6 +105,8 @@ int bin_finalize_jobs(const Job* jobs, int njobs, int max_nb, int G, hipStream_t
 // validates/derives the tile geometry; tile == nullptr -> default rows per tile;
 // tile[0] == 0 -> direct global atomics; otherwise tile[1] = rows per tile
 int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g, bool* direct);
+// particles per counting block (clamped to the supported range) and the block count
+void set_chunk(TileGeom* g, int64_t np, int64_t chunk);
 size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np);
 // points the sorted-deposit scratch of j into base (nullptr: sizes only); returns bytes
 size_t carve_scatter(Job& j, bool direct, void* base);

@@ -20,9 +20,18 @@ __device__ __forceinline__ int floor_mod(int a, int n) {
 
 __device__ __forceinline__ void load_particle(const ScatterArgs& a, int64_t i, float& px, float& py, float& pz,
                                               float& w) {
-  px = (a.x[i * a.sx] - a.xmin) * a.inv;
-  py = (a.y[i * a.sy] - a.ymin) * a.inv;
-  pz = (a.z[i * a.sz] - a.zmin) * a.inv;
+  if (a.aos) {   // one 12-byte load per particle
+    typedef float f3 __attribute__((ext_vector_type(3)));
+    f3 v;
+    __builtin_memcpy(&v, a.x + 3 * i, sizeof(float) * 3);
+    px = (v.x - a.xmin) * a.inv;
+    py = (v.y - a.ymin) * a.inv;
+    pz = (v.z - a.zmin) * a.inv;
+  } else {
+    px = (a.x[i * a.sx] - a.xmin) * a.inv;
+    py = (a.y[i * a.sy] - a.ymin) * a.inv;
+    pz = (a.z[i * a.sz] - a.zmin) * a.inv;
+  }
   w = a.w ? a.w[i * a.sw] : 1.0f;
 }
 

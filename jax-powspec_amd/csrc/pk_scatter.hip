@@ -48,14 +48,17 @@ constexpr int CNT_NT = 1024;
 // 1. count: LDS histogram of one chunk -> matrix row; FP64 weight sum per block
 // ------------------------------------------------------------------------------------
 __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
-  extern __shared__ unsigned hist2[];   // key k -> 16-bit half (k & 1) of word k >> 1
+  // chunks of <= 65535 particles: key k -> 16-bit half (k & 1) of word k >> 1 (half the
+  // LDS); larger chunks: one 32-bit counter per key
+  extern __shared__ unsigned hist[];
   __shared__ double red[CNT_NT];
   __shared__ unsigned wmax_bits;
   const ScatterArgs a = j.a;
   const TileGeom g = j.g;
   const int tid = threadIdx.x;
-  const int words = (g.nkeys + 1) >> 1;
-  for (int i = tid; i < words; i += CNT_NT) hist2[i] = 0u;
+  const bool packed = g.chunk <= 65535;
+  const int words = packed ? (g.nkeys + 1) >> 1 : g.nkeys;
+  for (int i = tid; i < words; i += CNT_NT) hist[i] = 0u;
   if (tid == 0) wmax_bits = 0u;
   __syncthreads();
   const int64_t lo = (int64_t)bx * g.chunk;
@@ -67,7 +70,8 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
     load_particle(a, i, px, py, pz, w);
     const int key = row_tile_key(a, g, px, py, pz);
     if (key >= 0) {
-      atomicAdd(&hist2[key >> 1], 1u << (16 * (key & 1)));
+      if (packed) atomicAdd(&hist[key >> 1], 1u << (16 * (key & 1)));
+      else atomicAdd(&hist[key], 1u);
       ws += (double)w;
       wm = fmaxf(wm, fabsf(w));
     }
@@ -76,7 +80,7 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
   __syncthreads();
   if (tid == 0) j.wmax[bx] = (double)__uint_as_float(wmax_bits);
   unsigned* row = j.mat + (size_t)bx * g.nkeys;
-  for (int k = tid; k < g.nkeys; k += CNT_NT) row[k] = (hist2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+  for (int k = tid; k < g.nkeys; k += CNT_NT) row[k] = packed ? (hist[k >> 1] >> (16 * (k & 1))) & 0xffffu : hist[k];
   const double t = block_sum_f64(ws, red, CNT_NT);
   if (tid == 0) j.wsum[bx] = t;
 }
@@ -96,8 +100,21 @@ __device__ __forceinline__ void col_scan_body(const Job& j, int bx) {
   unsigned* mat = j.mat;
   const int k = bx * COL_NT + threadIdx.x;
   if (k >= nkeys) return;
+  // rows in groups of 8: the 8 loads of a group are independent and stay in flight together
+  constexpr int UR = 8;
   unsigned run = 0;
-  for (int b = 0; b < nblk; ++b) {
+  int b = 0;
+  for (; b + UR <= nblk; b += UR) {
+    unsigned c[UR];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) c[u] = mat[(size_t)(b + u) * nkeys + k];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      mat[(size_t)(b + u) * nkeys + k] = run;
+      run += c[u];
+    }
+  }
+  for (; b < nblk; ++b) {
     unsigned* p = mat + (size_t)b * nkeys + k;
     const unsigned c = *p;
     *p = run;
@@ -374,12 +391,21 @@ __global__ void __launch_bounds__(ATM_NT) k_mass_finalize_jobs(const Job* __rest
 // host side
 // ------------------------------------------------------------------------------------
 constexpr size_t LDS_MAX = 160 * 1024;
-constexpr int CHUNK = 65535;
+constexpr int CHUNK_MIN = 4096;          // smallest counting chunk (small catalogues)
+constexpr int CHUNK_MAX = 1 << 22;       // largest (32-bit LDS counters above 65535)
 
 static int largest_divisor_le(int n, int cap) {
   for (int d = cap < n ? cap : n; d >= 2; --d)
     if (n % d == 0) return d;
   return n;
+}
+
+void set_chunk(TileGeom* g, int64_t np, int64_t chunk) {
+  if (chunk < CHUNK_MIN) chunk = CHUNK_MIN;
+  if (chunk > CHUNK_MAX) chunk = CHUNK_MAX;
+  g->chunk = (int)chunk;
+  g->nblk = (int)((np + chunk - 1) / chunk);
+  if (g->nblk < 1) g->nblk = 1;
 }
 
 int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g, bool* direct) {
@@ -415,8 +441,12 @@ int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g,
   g->ntiles = (int)ntiles;
   g->nkeys = (int)nkeys;
   g->fixed = (size_t)ty * n * 8 <= FIXED_LDS_MAX ? 1 : 0;
-  g->chunk = CHUNK;
-  g->nblk = (int)((np + CHUNK - 1) / CHUNK);
+  // one counting / placing block per CU when the catalogue is large (a second, partial
+  // round of blocks would cost a whole round), at least CHUNK_MIN particles per block
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int64_t ncu = num_cus(dev);
+  set_chunk(g, np, (np + ncu - 1) / ncu);
   if (g->nblk < 1) g->nblk = 1;
   if ((long)g->nkeys > (long)SCAN_BLOCK * SCAN_BLOCK || g->nblk > SCAN_BLOCK * SCAN_BLOCK)
     return set_error(PK_E_INVALID, "too many buckets");
@@ -446,7 +476,7 @@ int allow_lds(K kernel, size_t bytes) {
   return e;
 }
 
-size_t hist_lds(int nkeys) { return (size_t)((nkeys + 1) / 2) * 4; }
+size_t hist_lds(const TileGeom& g) { return g.chunk <= 65535 ? (size_t)((g.nkeys + 1) / 2) * 4 : (size_t)g.nkeys * 4; }
 size_t dest_lds(int nkeys) { return (size_t)nkeys * 4; }
 }  // namespace
 
@@ -517,7 +547,7 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
   }
   {
     StageTimer t(ST_SORT_COUNT, s);
-    const size_t lds = hist_lds(g.nkeys);
+    const size_t lds = hist_lds(g);
     int e = allow_lds(k_bucket_count, lds);
     if (e) return e;
     hipLaunchKernelGGL(k_bucket_count, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
@@ -582,7 +612,7 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
   }
   {
     StageTimer t(ST_SORT_COUNT, s);
-    const size_t lds = hist_lds(d.max_nkeys);
+    const size_t lds = (size_t)d.max_nkeys * 4;   // any job may count with 32-bit counters
     int e = allow_lds(k_bucket_count_jobs, lds);
     if (e) return e;
     hipLaunchKernelGGL(k_bucket_count_jobs, dim3(d.max_nblk, nj), dim3(CNT_NT), lds, s, jobs);

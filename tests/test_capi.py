@@ -68,3 +68,22 @@ def test_invalid_arguments_fail_before_any_launch(lib):
 def test_python_error_mapping(lib):
     with pytest.raises(ValueError):
         _lib.check("pk_scatter_workspace_size", lib.pk_scatter_workspace_size(1, 1, 0, None, None))
+
+
+def test_pk_job_layout_matches_the_header(tmp_path):
+    """ctypes' PkJob has the size and field offsets of struct pk_job as a C compiler lays it out."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = [f for f, _ in _lib.PkJob._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "pk_api.h"\nint main(void){\n'
+                   '  printf("%zu\\n", sizeof(pk_job));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(pk_job, {f}));\n' for f in fields) + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == C.sizeof(_lib.PkJob)
+    assert out[1:] == [getattr(_lib.PkJob, f).offset for f in fields]
