@@ -113,13 +113,14 @@ int pk_r2c(pk_plan* plan, const float* mesh, const double* stats, void* spectrum
  * Runs the z/y FFT passes into `spectrum` (caller buffer, (n/2+1) n^2 float2), then the
  * x pass fused with deconvolution (x / sin x)^(2p) per axis, |delta_k|^2, Legendre
  * weights and FP64 per-bin sums.  Outputs (device, nb entries; any may be NULL except
- * pk): sums[nb][3] FP64 (S0, S2, S4), counts[nb] int64, pk[nb][3] float32
+ * pk): sums[nb][3] FP64 (S0, S2, S4), counts[nb] int64 (a copy of modes), pk[nb][3] float32
  * = S_l / N * (2l+1) * (box/n^2)^3 with P_0 -= shot_noise (NaN for empty bins),
  * nmodes[nb] float32, kmean[nb] FP64 (needs PK_FLAG_KMEAN; units of k_F).
  */
 int pk_powspec_workspace_size(pk_plan* plan, int nb, size_t* bytes);
 int pk_powspec_mesh(pk_plan* plan, const float* mesh, const double* stats,
-                    const int32_t* thr, const int32_t* jstart, int nb, int window_p, int flags,
+                    const int32_t* thr, const int32_t* jstart, const int64_t* modes, const double* ksum,
+                    int nb, int window_p, int flags,
                     double box_size, double shot_noise, void* spectrum,
                     double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
                     void* workspace, size_t workspace_bytes, void* stream);
@@ -135,8 +136,8 @@ int pk_estimate(pk_plan* plan,
                 const float* w, int64_t sw, int64_t np,
                 float xmin, float ymin, float zmin, double box_size,
                 int stencil, int wrap_mode, const int* tile,
-                const int32_t* thr, const int32_t* jstart, int nb, int window_p, int flags,
-                double shot_noise,
+                const int32_t* thr, const int32_t* jstart, const int64_t* modes, const double* ksum,
+                int nb, int window_p, int flags, double shot_noise,
                 double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
                 double* stats, void* workspace, size_t workspace_bytes, void* stream);
 
@@ -189,7 +190,7 @@ typedef struct pk_job {
   float xmin, ymin, zmin;
   double box_size;
   int stencil, wrap_mode, window_p, flags;
-  const int32_t* thr; const int32_t* jstart; int nb;
+  const int32_t* thr; const int32_t* jstart; const int64_t* modes; const double* ksum; int nb;
   double shot_noise;
   double* sums; int64_t* counts; float* pk; float* nmodes; double* kmean;  /* as pk_estimate */
   double* stats;                          /* nullable: PK_STATS_LEN doubles */

@@ -44,7 +44,7 @@ class PkJob(C.Structure):
         ("w", _vp), ("sw", _i64), ("np", _i64),
         ("xmin", _f), ("ymin", _f), ("zmin", _f), ("box_size", _d),
         ("stencil", _i), ("wrap_mode", _i), ("window_p", _i), ("flags", _i),
-        ("thr", _vp), ("jstart", _vp), ("nb", _i), ("shot_noise", _d),
+        ("thr", _vp), ("jstart", _vp), ("modes", _vp), ("ksum", _vp), ("nb", _i), ("shot_noise", _d),
         ("sums", _vp), ("counts", _vp), ("pk", _vp), ("nmodes", _vp), ("kmean", _vp), ("stats", _vp),
     ]
 
@@ -60,11 +60,11 @@ SIGNATURES = [
                         _vp, _vp, _vp, _vp, _sz, _vp]),
     ("pk_r2c", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("pk_powspec_workspace_size", _i, [_vp, _i, _psz]),
-    ("pk_powspec_mesh", _i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                             _sz, _vp]),
+    ("pk_powspec_mesh", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp,
+                             _vp, _vp, _sz, _vp]),
     ("pk_estimate_workspace_size", _i, [_vp, _i64, _i, _vp, _i, _psz]),
     ("pk_estimate", _i, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _f, _f, _f, _d, _i, _i, _vp,
-                         _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+                         _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     ("pk_estimate_backward_workspace_size", _i, [_vp, _i64, _i, _psz]),
     ("pk_estimate_backward", _i, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _f, _f, _f, _d, _i, _i, _vp,
                                   _vp, _vp, _i, _i, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),

@@ -37,7 +37,8 @@ class _Estimate(torch.autograd.Function):
         pos_c = pos.detach().to(torch.float32).contiguous()
         w_c = None if w is None else w.detach().to(torch.float32).contiguous()
         npart = pos_c.shape[0]
-        thr, jst, nb = cfg["tables"]
+        tab = cfg["tables"]
+        nb = tab.nb
         out = _Outputs(nb, device, bool(cfg["flags"] & _lib.FLAG_KMEAN))
         stats = torch.zeros(_lib.STATS_LEN, dtype=torch.float64, device=device)
         lib = _lib.load()
@@ -50,8 +51,8 @@ class _Estimate(torch.autograd.Function):
             ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=device)
             _lib.check("pk_estimate", lib.pk_estimate(
                 plan.handle, base, 3, base + 4 if base else None, 3, base + 8 if base else None, 3, _device.ptr(w_c), 1,
-                npart, 0.0, 0.0, 0.0, cfg["box"], cfg["stencil"], cfg["wrap"], tptr, _device.ptr(thr),
-                _device.ptr(jst), nb, cfg["p"], cfg["flags"], cfg["shot"], _device.ptr(out.sums),
+                npart, 0.0, 0.0, 0.0, cfg["box"], cfg["stencil"], cfg["wrap"], tptr, _device.ptr(tab.thr),
+                _device.ptr(tab.jst), _device.ptr(tab.modes), _device.ptr(tab.ksum), nb, cfg["p"], cfg["flags"], cfg["shot"], _device.ptr(out.sums),
                 _device.ptr(out.counts), _device.ptr(out.pk), _device.ptr(out.nmodes), _device.ptr(out.kmean),
                 _device.ptr(stats), _device.ptr(ws), nbytes.value, _device.stream_ptr(device)))
         ctx.cfg = cfg
@@ -69,7 +70,8 @@ class _Estimate(torch.autograd.Function):
         device = pos_c.device
         n = cfg["n"]
         plan = _device.plan(n, device)
-        thr, jst, nb = cfg["tables"]
+        tab = cfg["tables"]
+        nb = tab.nb
         npart = pos_c.shape[0]
         if grad_pk is None:
             return None, None, None
@@ -88,7 +90,7 @@ class _Estimate(torch.autograd.Function):
             _lib.check("pk_estimate_backward", lib.pk_estimate_backward(
                 plan.handle, base, 3, base + 4 if base else None, 3, base + 8 if base else None, 3,
                 _device.ptr(w_c) if ctx.has_w else None, 1, npart, 0.0, 0.0, 0.0, cfg["box"], cfg["stencil"],
-                cfg["wrap"], tptr, _device.ptr(thr), _device.ptr(jst), nb, cfg["p"], _device.ptr(ws), ws.numel(),
+                cfg["wrap"], tptr, _device.ptr(tab.thr), _device.ptr(tab.jst), nb, cfg["p"], _device.ptr(ws), ws.numel(),
                 _device.ptr(stats), _device.ptr(gpk), _device.ptr(counts), _device.ptr(grad_pos),
                 _device.ptr(grad_w), _device.ptr(bws), nbytes.value, _device.stream_ptr(device)))
         return grad_pos, grad_w, None

@@ -67,7 +67,8 @@ class _Job:
         self.fundamental = k_edges is None
         self.edges = (binning.fundamental_grid_edges(n) if self.fundamental
                       else binning.grid_edges(k_edges, self.box))
-        self.thr, self.jst, self.nb = binning.TABLES.get(device, n, self.edges)
+        self.tables = binning.TABLES.get(device, n, self.edges)
+        self.nb = self.tables.nb
         window = cat.get("window")
         self.p = _window_p(window if window is not None else ("TSC" if mas == "TSC" else "CIC"))
         self.flags = _lib.FLAG_KMEAN if (self.fundamental or cat.get("full")) else 0
@@ -165,7 +166,10 @@ class BatchEstimator:
             q.wrap_mode = j.wrap
             q.window_p = j.p
             q.flags = j.flags
-            q.thr, q.jstart, q.nb = _device.ptr(j.thr), _device.ptr(j.jst), j.nb
+            t = j.tables
+            q.thr, q.jstart, q.modes, q.ksum = (_device.ptr(t.thr), _device.ptr(t.jst), _device.ptr(t.modes),
+                                                _device.ptr(t.ksum))
+            q.nb = j.nb
             q.shot_noise = j.shot
             q.sums = _device.ptr(self.sums[i])
             q.counts = _device.ptr(self.counts[i])

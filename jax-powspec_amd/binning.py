@@ -90,6 +90,39 @@ def slots_for_m(m: np.ndarray, thr: np.ndarray) -> np.ndarray:
     return np.where(ok, idx, -1)
 
 
+_MULT: dict = {}
+
+
+def mode_multiplicity(n: int) -> np.ndarray:
+    """mult[m] = number of stored half-spectrum entries (kx, ky in [0, n), kz in [0, n/2],
+    signed wavenumbers with Nyquist +n/2) with kx^2 + ky^2 + kz^2 = m; cached per n."""
+    hit = _MULT.get(n)
+    if hit is not None:
+        return hit
+    ki = np.arange(n)
+    ki = np.where(ki > n // 2, ki - n, ki).astype(np.int64)
+    hist_xy = np.bincount((ki[:, None] ** 2 + ki[None, :] ** 2).reshape(-1))
+    mult = np.zeros(max_m(n) + 1, np.int64)
+    for kz in range(n // 2 + 1):
+        mult[kz * kz:kz * kz + hist_xy.size] += hist_xy
+    _MULT[n] = mult
+    return mult
+
+
+def bin_modes(thr: np.ndarray, n: int):
+    """Exact per-bin mode counts (int64) and sums of |k| in units of k_F (FP64) over the
+    stored half spectrum, from the integer thresholds: every entry counts once, the k = 0
+    entry included when its bin exists."""
+    mult = mode_multiplicity(n)
+    nb = thr.size - 1
+    m = np.arange(mult.size, dtype=np.int64)
+    slot = slots_for_m(m, thr)
+    sel = slot >= 0
+    counts = np.bincount(slot[sel], weights=mult[sel], minlength=nb)[:nb]
+    ksum = np.bincount(slot[sel], weights=mult[sel] * np.sqrt(m[sel].astype(np.float64)), minlength=nb)[:nb]
+    return np.rint(counts).astype(np.int64), ksum
+
+
 def fundamental_grid_edges(n: int) -> np.ndarray:
     """Edges 1, 2, ..., kmax+1 (k_F units): bin j = [j, j+1) for j >= 1, the slots of
     src/correlations.py:95-109 after the DC slot is dropped; kmax = int32(sqrt(3 mid^2))."""
@@ -105,12 +138,24 @@ def grid_edges(k_edges, box_size: float) -> np.ndarray:
     return np.asarray(k_edges, np.float64).reshape(-1) / kF
 
 
-class TableCache:
-    """Small LRU of device-resident (thr, jstart) tensors keyed by (device, n, edges)."""
+class BinTable:
+    """Device-resident bin tables of one (n, edges): thr int32[nb+1], jst int32, exact mode
+    counts int64[nb] and sums of |k| (k_F units) FP64[nb]."""
 
-    def __init__(self, capacity: int = 64):
+    def __init__(self, thr, jst, nb, modes, ksum):
+        self.thr, self.jst, self.nb, self.modes, self.ksum = thr, jst, nb, modes, ksum
+
+
+class TableCache:
+    """Bin tables keyed by (device, n, edges).  Edge arrays that give the same integer
+    thresholds (e.g. the same k_F multiples from different box sizes) share one device
+    table; both levels are LRUs."""
+
+    def __init__(self, capacity: int = 4096, tables: int = 256):
         self.capacity = capacity
+        self.tables = tables
         self._d: OrderedDict = OrderedDict()
+        self._t: OrderedDict = OrderedDict()
 
     def get(self, device, n: int, edges: np.ndarray):
         import torch
@@ -120,7 +165,17 @@ class TableCache:
             self._d.move_to_end(key)
             return hit
         thr, jst = thresholds_from_grid_edges(edges, n)
-        val = (torch.from_numpy(thr).to(device), torch.from_numpy(jst).to(device), thr.size - 1)
+        tkey = (str(device), int(n), thr.tobytes())
+        val = self._t.get(tkey)
+        if val is None:
+            modes, ksum = bin_modes(thr, n)
+            val = BinTable(torch.from_numpy(thr).to(device), torch.from_numpy(jst).to(device), thr.size - 1,
+                           torch.from_numpy(modes).to(device), torch.from_numpy(ksum).to(device))
+            self._t[tkey] = val
+            if len(self._t) > self.tables:
+                self._t.popitem(last=False)
+        else:
+            self._t.move_to_end(tkey)
         self._d[key] = val
         if len(self._d) > self.capacity:
             self._d.popitem(last=False)

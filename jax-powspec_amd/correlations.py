@@ -60,7 +60,8 @@ def _powspec(delta, box_size, edges_grid, *, p, shot_noise, flags, stats=None):
     mesh = _device.as_mesh(delta, device)
     n = mesh.shape[0]
     plan = _device.plan(n, device)
-    thr, jst, nb = binning.TABLES.get(device, n, edges_grid)
+    tab = binning.TABLES.get(device, n, edges_grid)
+    nb = tab.nb
     lib = _lib.load()
     out = _Outputs(nb, device, bool(flags & _lib.FLAG_KMEAN))
     nbytes = C.c_size_t()
@@ -69,7 +70,8 @@ def _powspec(delta, box_size, edges_grid, *, p, shot_noise, flags, stats=None):
         ws = _device.workspace(nbytes.value, device)
         spec = torch.empty((n // 2 + 1, n, n, 2), dtype=torch.float32, device=device)
         _lib.check("pk_powspec_mesh", lib.pk_powspec_mesh(
-            plan.handle, _device.ptr(mesh), _device.ptr(stats), _device.ptr(thr), _device.ptr(jst), nb, p, flags,
+            plan.handle, _device.ptr(mesh), _device.ptr(stats), _device.ptr(tab.thr), _device.ptr(tab.jst),
+            _device.ptr(tab.modes), _device.ptr(tab.ksum), nb, p, flags,
             float(box_size), float(shot_noise), _device.ptr(spec), _device.ptr(out.sums), _device.ptr(out.counts),
             _device.ptr(out.pk), _device.ptr(out.nmodes), _device.ptr(out.kmean), _device.ptr(ws), nbytes.value,
             _device.stream_ptr(device)))
@@ -161,7 +163,8 @@ def estimate(x, y, z, w=None, *, box_size, n_bins, k_edges=None, mas="CIC_REF", 
         ws_, sw = _device.as_f32_vector(w, device, "w")
     fundamental = k_edges is None
     e = binning.fundamental_grid_edges(n) if fundamental else binning.grid_edges(k_edges, box_size)
-    thr, jst, nb = binning.TABLES.get(device, n, e)
+    tab = binning.TABLES.get(device, n, e)
+    nb = tab.nb
     p = _window_p(window if window is not None else ("TSC" if mas == "TSC" else "CIC"))
     flags = _lib.FLAG_KMEAN if (full or fundamental) else 0
     out = _Outputs(nb, device, bool(flags & _lib.FLAG_KMEAN))
@@ -176,8 +179,8 @@ def estimate(x, y, z, w=None, *, box_size, n_bins, k_edges=None, mas="CIC_REF", 
         _lib.check("pk_estimate", lib.pk_estimate(
             plan.handle, _device.ptr(xs), sx, _device.ptr(ys), sy, _device.ptr(zs), sz, _device.ptr(ws_), sw, np_,
             _device.as_f32_scalar(xmin), _device.as_f32_scalar(ymin), _device.as_f32_scalar(zmin), float(box_size),
-            stencil, _lib.WRAP_VEC_TRUE if wrap else _lib.WRAP_VEC_FALSE, tptr, _device.ptr(thr),
-            _device.ptr(jst), nb, p, flags, float(shot_noise), _device.ptr(out.sums), _device.ptr(out.counts),
+            stencil, _lib.WRAP_VEC_TRUE if wrap else _lib.WRAP_VEC_FALSE, tptr, _device.ptr(tab.thr),
+            _device.ptr(tab.jst), _device.ptr(tab.modes), _device.ptr(tab.ksum), nb, p, flags, float(shot_noise), _device.ptr(out.sums), _device.ptr(out.counts),
             _device.ptr(out.pk), _device.ptr(out.nmodes), _device.ptr(out.kmean), _device.ptr(stats),
             _device.ptr(ws), nbytes.value, _device.stream_ptr(device)))
     kF = 2.0 * np.pi / float(box_size)
@@ -211,7 +214,8 @@ class Estimator:
         self.fundamental = k_edges is None
         self.edges = (binning.fundamental_grid_edges(self.n) if self.fundamental
                       else binning.grid_edges(k_edges, self.box))
-        self.thr, self.jst, self.nb = binning.TABLES.get(self.device, self.n, self.edges)
+        self.tables = binning.TABLES.get(self.device, self.n, self.edges)
+        self.nb = self.tables.nb
         self.p = _window_p(window if window is not None else ("TSC" if self.mas == "TSC" else "CIC"))
         self.flags = _lib.FLAG_KMEAN if (want_k or self.fundamental) else 0
         self.out = _Outputs(self.nb, self.device, bool(self.flags & _lib.FLAG_KMEAN))
@@ -244,8 +248,8 @@ class Estimator:
         o = self.out
         _lib.check("pk_estimate", _lib.load().pk_estimate(
             self.plan.handle, _device.ptr(xs), sx, _device.ptr(ys), sy, _device.ptr(zs), sz, _device.ptr(ws_), sw,
-            np_, 0.0, 0.0, 0.0, self.box, self.stencil, self.wrap_mode, self._tile_ptr, _device.ptr(self.thr),
-            _device.ptr(self.jst), self.nb, self.p, self.flags, self.shot_noise, _device.ptr(o.sums),
+            np_, 0.0, 0.0, 0.0, self.box, self.stencil, self.wrap_mode, self._tile_ptr, _device.ptr(self.tables.thr),
+            _device.ptr(self.tables.jst), _device.ptr(self.tables.modes), _device.ptr(self.tables.ksum), self.nb, self.p, self.flags, self.shot_noise, _device.ptr(o.sums),
             _device.ptr(o.counts), _device.ptr(o.pk), _device.ptr(o.nmodes), _device.ptr(o.kmean),
             _device.ptr(self.stats), _device.ptr(self.ws), self.ws_bytes, _device.stream_ptr(self.device, stream)))
         return o.pk

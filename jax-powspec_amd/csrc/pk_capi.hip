@@ -159,26 +159,30 @@ int make_args(ScatterArgs* a, const float* x, int64_t sx, const float* y, int64_
 // workspace of pk_powspec_mesh: bin partials + z-pass DC partials (spectrum is an argument)
 size_t powspec_ws_bytes(int n, int nb) {
   const int G = bin_grid(n, nb);
-  return align_up((size_t)G * (size_t)(nb > 0 ? nb : 1) * 5 * sizeof(double)) +
+  return align_up((size_t)G * (size_t)(nb > 0 ? nb : 1) * 3 * sizeof(double)) +
          align_up((size_t)zpass_grid(n) * sizeof(double));
 }
 
-int check_bins(const int32_t* thr, const int32_t* jstart, int nb, const float* pk) {
+int check_bins(const int32_t* thr, const int32_t* jstart, const int64_t* modes, const double* ksum, int nb,
+               int flags, const float* pk) {
   if (nb < 1) return set_error(PK_E_INVALID, "need at least one bin (nb=%d)", nb);
   if (nb > (1 << 20)) return set_error(PK_E_INVALID, "too many bins (nb=%d)", nb);
-  if (!thr || !jstart || !pk) return set_error(PK_E_INVALID, "null threshold / jstart / pk pointer");
+  if (!thr || !jstart || !modes || !pk) return set_error(PK_E_INVALID, "null threshold / jstart / modes / pk pointer");
+  if ((flags & PK_FLAG_KMEAN) && !ksum) return set_error(PK_E_INVALID, "PK_FLAG_KMEAN needs the ksum table");
   return PK_OK;
 }
 
 // binning fields of a job (everything the y pass, x pass + bin and finalize read)
-void set_bin_fields(Job& j, const pk_plan* p, float2* spec, const int32_t* thr, const int32_t* jstart, int nb,
-                    int window_p, int flags, double box, double shot, double* sums, int64_t* counts, float* pk,
-                    float* nmodes, double* kmean) {
+void set_bin_fields(Job& j, const pk_plan* p, float2* spec, const int32_t* thr, const int32_t* jstart,
+                    const int64_t* modes, const double* ksum, int nb, int window_p, int flags, double box,
+                    double shot, double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean) {
   const int n = p->n;
   j.spec = spec;
   j.wtab = p->wtab + (size_t)window_p * n;
   j.thr = thr;
   j.jst = jstart;
+  j.modes = modes;
+  j.ksum = ksum;
   j.nb = nb;
   j.flags = flags;
   const double h = box / ((double)n * (double)n);
@@ -204,11 +208,12 @@ int finish_powspec(pk_plan* p, const Job& j, hipStream_t s) {
 
 double* dc_partials(void* ws, int n, int nb) {
   return reinterpret_cast<double*>(static_cast<char*>(ws) +
-                                   align_up((size_t)bin_grid(n, nb) * (size_t)nb * 5 * sizeof(double)));
+                                   align_up((size_t)bin_grid(n, nb) * (size_t)nb * 3 * sizeof(double)));
 }
 
 int run_powspec(pk_plan* p, const float* mesh, const double* stats, const int32_t* thr, const int32_t* jstart,
-                int nb, int window_p, int flags, double box, double shot, float2* spec, double* sums,
+                const int64_t* modes, const double* ksum, int nb, int window_p, int flags, double box, double shot,
+                float2* spec, double* sums,
                 int64_t* counts, float* pk, float* nmodes, double* kmean, void* ws, size_t ws_bytes,
                 hipStream_t s) {
   const int n = p->n;
@@ -217,7 +222,8 @@ int run_powspec(pk_plan* p, const float* mesh, const double* stats, const int32_
   if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
   Job j{};
   j.active = 1;
-  set_bin_fields(j, p, spec, thr, jstart, nb, window_p, flags, box, shot, sums, counts, pk, nmodes, kmean);
+  set_bin_fields(j, p, spec, thr, jstart, modes, ksum, nb, window_p, flags, box, shot, sums, counts, pk, nmodes,
+                 kmean);
   j.partials = static_cast<double*>(ws);
   // with the contrast applied (ZERO_DC) the k = 0 mode is exactly 0 and no DC sum is needed
   const bool keep_dc = (flags & PK_FLAG_ZERO_DC) == 0;
@@ -330,16 +336,16 @@ int pk_powspec_workspace_size(pk_plan* p, int nb, size_t* bytes) {
 }
 
 int pk_powspec_mesh(pk_plan* p, const float* mesh, const double* stats, const int32_t* thr, const int32_t* jstart,
-                    int nb, int window_p, int flags, double box_size, double shot_noise, void* spectrum,
-                    double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean, void* workspace,
-                    size_t workspace_bytes, void* stream) {
+                    const int64_t* modes, const double* ksum, int nb, int window_p, int flags, double box_size,
+                    double shot_noise, void* spectrum, double* sums, int64_t* counts, float* pk, float* nmodes,
+                    double* kmean, void* workspace, size_t workspace_bytes, void* stream) {
   int e = check_plan(p);
   if (e) return e;
-  e = check_bins(thr, jstart, nb, pk);
+  e = check_bins(thr, jstart, modes, ksum, nb, flags, pk);
   if (e) return e;
   if (!mesh || !spectrum) return set_error(PK_E_INVALID, "null mesh / spectrum");
   if (!(box_size > 0.0)) return set_error(PK_E_INVALID, "box_size must be > 0");
-  return run_powspec(p, mesh, stats, thr, jstart, nb, window_p, flags, box_size, shot_noise,
+  return run_powspec(p, mesh, stats, thr, jstart, modes, ksum, nb, window_p, flags, box_size, shot_noise,
                      static_cast<float2*>(spectrum), sums, counts, pk, nmodes, kmean, workspace, workspace_bytes,
                      as_stream(stream));
 }
@@ -399,13 +405,13 @@ int pk_estimate_workspace_size(pk_plan* p, int64_t np, int stencil, const int* t
 
 int pk_estimate(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t sy, const float* z, int64_t sz,
                 const float* w, int64_t sw, int64_t np, float xmin, float ymin, float zmin, double box_size,
-                int stencil, int wrap_mode, const int* tile, const int32_t* thr, const int32_t* jstart, int nb,
-                int window_p, int flags, double shot_noise, double* sums, int64_t* counts, float* pk,
-                float* nmodes, double* kmean, double* stats, void* workspace, size_t workspace_bytes,
-                void* stream) {
+                int stencil, int wrap_mode, const int* tile, const int32_t* thr, const int32_t* jstart,
+                const int64_t* modes, const double* ksum, int nb, int window_p, int flags, double shot_noise,
+                double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean, double* stats,
+                void* workspace, size_t workspace_bytes, void* stream) {
   int e = check_plan(p);
   if (e) return e;
-  e = check_bins(thr, jstart, nb, pk);
+  e = check_bins(thr, jstart, modes, ksum, nb, flags, pk);
   if (e) return e;
   ScatterArgs a{};
   e = make_args(&a, x, sx, y, sy, z, sz, w, sw, np, xmin, ymin, zmin, box_size, p->n, stencil, wrap_mode);
@@ -426,14 +432,15 @@ int pk_estimate(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t 
   if (!ws.fused) {
     e = scatter_run(j, direct, nullptr, ws.mesh, ws.scatter_ws, ws.scatter_bytes, s);
     if (e) return e;
-    return run_powspec(p, ws.mesh, j.stats, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size,
-                       shot_noise, ws.spec, sums, counts, pk, nmodes, kmean, ws.powspec_ws, ws.powspec_bytes, s);
+    return run_powspec(p, ws.mesh, j.stats, thr, jstart, modes, ksum, nb, window_p, flags | PK_FLAG_ZERO_DC,
+                       box_size, shot_noise, ws.spec, sums, counts, pk, nmodes, kmean, ws.powspec_ws,
+                       ws.powspec_bytes, s);
   }
   // fused: sort -> deposit + z pass -> mass -> y pass -> x pass + bin -> finalize scaled by (n^3 / M)^2
   e = scatter_run(j, false, nullptr, nullptr, ws.scatter_ws, ws.scatter_bytes, s);
   if (e) return e;
-  set_bin_fields(j, p, ws.spec, thr, jstart, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size, shot_noise, sums,
-                 counts, pk, nmodes, kmean);
+  set_bin_fields(j, p, ws.spec, thr, jstart, modes, ksum, nb, window_p, flags | PK_FLAG_ZERO_DC, box_size,
+                 shot_noise, sums, counts, pk, nmodes, kmean);
   j.partials = static_cast<double*>(ws.powspec_ws);
   j.scale_stats = j.stats;
   e = fused_deposit_zpass(p->n, j, p->tw, s);
@@ -515,8 +522,8 @@ int pk_estimate_backward(pk_plan* p, const float* x, int64_t sx, const float* y,
   if (e) return e;
   Job j{};
   j.active = 1;
-  set_bin_fields(j, p, fw.spec, thr, jstart, nb, window_p, PK_FLAG_ZERO_DC, box_size, 0.0, nullptr, nullptr, nullptr,
-                 nullptr, nullptr);
+  set_bin_fields(j, p, fw.spec, thr, jstart, counts, nullptr, nb, window_p, PK_FLAG_ZERO_DC, box_size, 0.0, nullptr,
+                 nullptr, nullptr, nullptr, nullptr);
   j.scale_stats = fw.fused ? stats : nullptr;   // fused: the spectrum is of rho - c, delta_k = S spec_k
   e = fft_adjoint(n, j, p->tw, bw.coef, bw.bspec, bw.gmesh, s);
   if (e) return e;
@@ -588,7 +595,7 @@ SlotLayout slot_layout(int n, int64_t max_np, int max_nb, const TileGeom& cap, i
   L.spec = 0;
   size_t off = align_up((size_t)(n / 2 + 1) * n * n * sizeof(float2));
   L.partials = off;
-  off += align_up((size_t)gmax * (size_t)max_nb * 5 * sizeof(double));
+  off += align_up((size_t)gmax * (size_t)max_nb * 3 * sizeof(double));
   L.scatter = off;
   Job cj{};
   cj.g = cap;
@@ -720,7 +727,7 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
                   q.stencil, q.wrap_mode);
     if (!e && q.np > b->max_np) e = set_error(PK_E_INVALID, "job %d: %lld particles > batch capacity %lld", i,
                                               (long long)q.np, (long long)b->max_np);
-    if (!e) e = check_bins(q.thr, q.jstart, q.nb, q.pk);
+    if (!e) e = check_bins(q.thr, q.jstart, q.modes, q.ksum, q.nb, q.flags, q.pk);
     if (!e && q.nb > b->max_nb) e = set_error(PK_E_INVALID, "job %d: %d bins > batch capacity %d", i, q.nb, b->max_nb);
     if (!e && (q.window_p < 0 || q.window_p > 3)) e = set_error(PK_E_INVALID, "job %d: window p=%d", i, q.window_p);
     bool direct = false;
@@ -736,8 +743,8 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     }
     j.active = 1;
     if (q.stats) j.stats = q.stats;
-    set_bin_fields(j, b->plan, j.spec, q.thr, q.jstart, q.nb, q.window_p, q.flags | PK_FLAG_ZERO_DC, q.box_size,
-                   q.shot_noise, q.sums, q.counts, q.pk, q.nmodes, q.kmean);
+    set_bin_fields(j, b->plan, j.spec, q.thr, q.jstart, q.modes, q.ksum, q.nb, q.window_p, q.flags | PK_FLAG_ZERO_DC,
+                   q.box_size, q.shot_noise, q.sums, q.counts, q.pk, q.nmodes, q.kmean);
     j.dc_part = nullptr;
     j.Gz = 0;
     j.scale_stats = j.stats;

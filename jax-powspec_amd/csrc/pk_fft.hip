@@ -201,17 +201,13 @@ constexpr int DPP_WAVE_SHL1 = 0x130; // lane l reads lane l + 1
 // one step of a segmented inclusive scan: (v, f) <- (v + v_src, f_src) where this lane's
 // run has not started yet (f == 0); a lane without a source reads (0, 0) and is unchanged
 template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step(float& a, float& b, float& c, float& d, float& e, int& f, bool with_e) {
+__device__ __forceinline__ void seg_step(float& a, float& b, float& c, int& f) {
   const float pa = dpp_move<CTRL, ROWS>(a), pb = dpp_move<CTRL, ROWS>(b), pc = dpp_move<CTRL, ROWS>(c);
-  const float pd = dpp_move<CTRL, ROWS>(d);
-  const float pe = with_e ? dpp_move<CTRL, ROWS>(e) : 0.0f;
   const int pf = dpp_move<CTRL, ROWS>(f);
   if (!f) {
     a += pa;
     b += pb;
     c += pc;
-    d += pd;
-    e += pe;
     f = pf;
   }
 }
@@ -223,7 +219,7 @@ __device__ __forceinline__ void seg_step(float& a, float& b, float& c, float& d,
 // the two cross-row steps (row_bcast15 into rows 1 and 3, then row_bcast31 into rows 2
 // and 3) of a full-wave scan.  Float partial sums of at most 16 terms (depth-4 tree:
 // relative error <= 4 ulp of the run's absolute sum).  Must be called by all 64 lanes.
-__device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, float& c, float& d, float& e, bool with_e) {
+__device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, float& c) {
   const int lane = threadIdx.x & 63;
   // every cross-lane move runs with all 64 lanes active (a lane the exec mask leaves out
   // reads as 0 to its neighbours), so none sits behind a short-circuit or a branch
@@ -231,15 +227,15 @@ __device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, floa
   int f = (lane == 0 || prev != slot) ? 1 : 0;               // this lane starts a run
   const int next_starts = dpp_move<DPP_WAVE_SHL1>(f);
   const bool tail = ((lane & 15) == 15) | (next_starts != 0);
-  seg_step<DPP_ROW_SHR + 1, 0xf>(a, b, c, d, e, f, with_e);
-  seg_step<DPP_ROW_SHR + 2, 0xf>(a, b, c, d, e, f, with_e);
-  seg_step<DPP_ROW_SHR + 4, 0xf>(a, b, c, d, e, f, with_e);
-  seg_step<DPP_ROW_SHR + 8, 0xf>(a, b, c, d, e, f, with_e);
+  seg_step<DPP_ROW_SHR + 1, 0xf>(a, b, c, f);
+  seg_step<DPP_ROW_SHR + 2, 0xf>(a, b, c, f);
+  seg_step<DPP_ROW_SHR + 4, 0xf>(a, b, c, f);
+  seg_step<DPP_ROW_SHR + 8, 0xf>(a, b, c, f);
   return tail;
 }
 
 constexpr int HB_LDS = 256;   // bins held in LDS; above this the per-block slice lives in HBM
-constexpr int NACC = 5;       // S0, S2, S4, count, sum sqrt(m)
+constexpr int NACC = 3;       // S0, S2, S4 (mode counts and sums of |k| come from the host tables)
 
 // ------------------------------------------------------------------------------------
 // x pass fused with deconvolution, |delta_k|^2, Legendre weights and per-bin FP64 sums.
@@ -250,7 +246,8 @@ constexpr int NACC = 5;       // S0, S2, S4, count, sum sqrt(m)
 // budget is held to that so registers never cost a resident block
 template <int N> struct XBinLds {
   static constexpr size_t TILE = (size_t)N * (FFTCfg<N>::CX + 1) * 8;
-  static constexpr size_t HIST = (size_t)(HB_LDS + 1) * 4 + 4 * HB_LDS * 8 + HB_LDS * 4;
+  static constexpr int JLEN = isqrt_c(3 * (N / 2) * (N / 2)) + 2;   // isqrt-start table entries
+  static constexpr size_t HIST = (size_t)(HB_LDS + 1) * 4 + 3 * HB_LDS * 8 + (size_t)(N / 2 + 1) * 4 + (size_t)JLEN * 4;
   // LDS-limited waves per SIMD (the histogram in HBM: no limit is imposed)
   template <bool LH> static constexpr int wpe() {
     return LH ? cmax(1, cmin(4, (int)((160 * 1024) / (TILE + HIST)) * (FFTCfg<N>::NTX / 64) / 4)) : 1;
@@ -264,38 +261,39 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   const int* __restrict__ thr_g = j.thr;
   const int* __restrict__ jst_g = j.jst;
   const int nb = j.nb;
-  const int flags = j.flags;
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
   constexpr int YT = N / C;
   constexpr int NTILES = (N / 2 + 1) * YT;
   constexpr int H = N / 2;
-  // LDS: the tile, the S_0,2,4 / count / sum-k histogram and the thresholds (<= 80 KB at
-  // N = 512 so two blocks share a CU); twiddles, window and isqrt-start tables are read
-  // through the caches
+  // LDS: the tile, the S_0,2,4 histogram, the thresholds, the window factors of |k| <= N/2
+  // and the isqrt-start table (<= 80 KB at N = 512 so two blocks share a CU); twiddles are
+  // read through the caches
+  constexpr int JLEN = XBinLds<N>::JLEN;
   __shared__ float2 buf[N * PS];
   __shared__ int thr_s[LH ? HB_LDS + 1 : 1];
   __shared__ double hs[LH ? 3 * HB_LDS : 1];
-  __shared__ unsigned int hc[LH ? HB_LDS : 1];
-  __shared__ double hk[LH ? HB_LDS : 1];
+  __shared__ float wt_s[LH ? H + 1 : 1];
+  __shared__ int jst_s[LH ? JLEN : 1];
   const float2* tw = twg;
-  const float* wt = wtab;
-  const int* jst = jst_g;
   const int tid = threadIdx.x;
   double* gpart = j.partials + (size_t)bx * nb * NACC;
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
     for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0.0;
-    for (int i = tid; i < HB_LDS; i += NT) {
-      hc[i] = 0u;
-      hk[i] = 0.0;
-    }
+    for (int i = tid; i <= H; i += NT) wt_s[i] = wtab[i];
+    for (int i = tid; i < JLEN; i += NT) jst_s[i] = jst_g[i];
   } else {
     for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;
   }
   __syncthreads();
   const int* thr = LH ? thr_s : thr_g;
-  const bool want_k = (flags & PK_FLAG_KMEAN) != 0;
+  const int* jst = LH ? jst_s : jst_g;
+  // window factor of array index i (symmetric: index N - i has the same |k|)
+  auto wt = [&](int i) -> float {
+    if constexpr (LH) return wt_s[i <= H ? i : N - i];
+    else return wtab[i];
+  };
   // pair tasks: |kx| = q in [0, N/2], column c; entries q and N-q share m and mu.  Every
   // thread keeps one column (NT is a multiple of C) and walks q in steps of NT / C, so the
   // loop is wave-uniform (idle lanes carry slot -1) and the wave can combine lanes that
@@ -304,9 +302,10 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   constexpr int QSTEP = NT / C;
   constexpr int NIT = (H + 1 + QSTEP - 1) / QSTEP;
   const int c = tid % C, q0 = tid / C;
-  // N = 64, 128: the next tile's columns are loaded into registers while this one is
-  // transformed and binned (for other N the registers would spill or cost a block)
-  constexpr bool XPF = LH && (N == 64 || N == 128);
+  // N = 128: the next tile's columns are loaded into registers while this one is transformed
+  // and binned (for larger N the registers would spill or cost a block; at N = 64 a block
+  // has a single tile)
+  constexpr bool XPF = LH && N == 128;
   constexpr int XPER = (N * C + NT - 1) / NT;
   float2 xpre[XPF ? XPER : 1];
   auto load_tile = [&](int t) {
@@ -329,8 +328,8 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     // bins and window factors do not depend on the transform: the first iteration's table
     // reads are issued before the tile's load and FFT, each later one an iteration ahead
     int slot_next = q0 <= H ? find_bin(q0 * q0 + r2, thr, jst, nb) : -1;
-    float wq_next = q0 <= H ? wt[q0] : 0.0f;
-    const float wyz = wt[y] * wt[kz];
+    float wq_next = q0 <= H ? wt(q0) : 0.0f;
+    const float wyz = wt(y) * wt(kz);
     if constexpr (XPF) {
 #pragma unroll
       for (int k = 0; k < XPER; ++k) {
@@ -353,17 +352,15 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       const float wqi = wq_next;
       const int qn = q + QSTEP;
       slot_next = qn <= H ? find_bin(qn * qn + r2, thr, jst, nb) : -1;
-      wq_next = qn <= H ? wt[qn] : 0.0f;
-      float p2 = 0.0f, v2 = 0.0f, v4 = 0.0f, vc = 0.0f, vk = 0.0f;
+      wq_next = qn <= H ? wt(qn) : 0.0f;
+      float p2 = 0.0f, v2 = 0.0f, v4 = 0.0f;
       if (slot >= 0) {
         const int m = q * q + r2;
         const float2 a = buf[q * PS + c];
         p2 = a.x * a.x + a.y * a.y;
-        int cnt = 1;
         if (q != 0 && q != H) {
           const float2 bb = buf[(N - q) * PS + c];
           p2 += bb.x * bb.x + bb.y * bb.y;
-          cnt = 2;
         }
         p2 *= wqi * wyz;
         // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
@@ -380,25 +377,19 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
         } else {
           p2 = 0.0f;
         }
-        vc = (float)cnt;
-        vk = want_k ? (float)cnt * sqrtf((float)m) : 0.0f;
       }
       // runs of equal slot in lane order -> one histogram update per run (its last lane)
-      const bool tail = wave_run_sums(slot, p2, v2, v4, vc, vk, want_k);
+      const bool tail = wave_run_sums(slot, p2, v2, v4);
       if (!tail || slot < 0) continue;
       if constexpr (LH) {
         atomicAdd(&hs[slot], (double)p2);
         atomicAdd(&hs[HB_LDS + slot], (double)v2);
         atomicAdd(&hs[2 * HB_LDS + slot], (double)v4);
-        atomicAdd(&hc[slot], (unsigned)vc);
-        if (want_k) atomicAdd(&hk[slot], (double)vk);
       } else {
         double* g = gpart + (size_t)slot * NACC;
         atomicAdd(g + 0, (double)p2);
         atomicAdd(g + 1, (double)v2);
         atomicAdd(g + 2, (double)v4);
-        atomicAdd(g + 3, (double)vc);
-        if (want_k) atomicAdd(g + 4, (double)vk);
       }
     }
     __syncthreads();
@@ -409,8 +400,6 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       g[0] = hs[i];
       g[1] = hs[HB_LDS + i];
       g[2] = hs[2 * HB_LDS + i];
-      g[3] = (double)hc[i];
-      g[4] = hk[i];
     }
   }
 }
@@ -448,7 +437,7 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   float* __restrict__ pk = j.pk;
   float* __restrict__ nmodes = j.nmodes;
   double* __restrict__ kmean = (j.flags & PK_FLAG_KMEAN) ? j.kmean : nullptr;
-  double part[NACC] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double part[NACC] = {0.0, 0.0, 0.0};
   for (int g = threadIdx.x; g < G; g += FIN_NT) {
     const double* p = partials + ((size_t)g * nb + b) * NACC;
 #pragma unroll
@@ -473,19 +462,20 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   acc[0] *= sc;
   acc[1] *= sc;
   acc[2] *= sc;
-  const double cnt = acc[3];
+  const int64_t nmode = j.modes[b];   // exact lattice count from the host tables
+  const double cnt = (double)nmode;
   if (sums) {
     sums[b * 3 + 0] = acc[0];
     sums[b * 3 + 1] = acc[1];
     sums[b * 3 + 2] = acc[2];
   }
-  if (counts) counts[b] = (int64_t)cnt;
+  if (counts) counts[b] = nmode;
   if (nmodes) nmodes[b] = (float)cnt;
   // 0/0 -> NaN for empty bins, as the reference (:50-52)
   pk[b * 3 + 0] = (float)(acc[0] / cnt * norm - shot);
   pk[b * 3 + 1] = (float)(acc[1] / cnt * 5.0 * norm);
   pk[b * 3 + 2] = (float)(acc[2] / cnt * 9.0 * norm);
-  if (kmean) kmean[b] = acc[4] / cnt;
+  if (kmean) kmean[b] = j.ksum[b] / cnt;
 }
 
 __global__ void __launch_bounds__(FIN_NT) k_bin_finalize(Job j, int G) { bin_finalize_body(j, G, blockIdx.x); }

@@ -28,7 +28,9 @@ template <int N, int TY_> struct FusedCfg {
   static constexpr int NC = N / 2;
   static constexpr int CS = NC + 1;                      // complex row stride in LDS
   static constexpr int TY = TY_;
-  static constexpr int NT = cmin(1024, cmax(64, (TY * NC / 8) / 64 * 64));
+  // at least 512 threads: small meshes have dense tiles, and the deposit of their records
+  // needs the parallelism even where the z transform alone would use fewer threads
+  static constexpr int NT = cmin(1024, cmax(cmin(512, TY * N), (TY * NC / 8) / 64 * 64));
   // deposit into 64-bit fixed-point cells when they fit (the FFT rows alias them afterwards)
   static constexpr bool FIXED = (size_t)TY * N * 8 <= FIXED_LDS_MAX;
   static constexpr size_t FFT_LDS = (size_t)TY * CS * sizeof(float2);

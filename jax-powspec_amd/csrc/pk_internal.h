@@ -66,9 +66,11 @@ struct Job {
   const float* wtab;           // window table of this job's exponent
   const int* thr;              // [nb + 1]
   const int* jst;              // isqrt start table
+  const int64_t* modes;        // [nb] exact mode counts
+  const double* ksum;          // [nb] sums of sqrt(m) (nullable without PK_FLAG_KMEAN)
   int nb;
   int flags;
-  double* partials;            // [G][nb][NACC] x-pass block partials
+  double* partials;            // [G][nb][3] x-pass block partials
   const double* dc_part;       // FP64 k = 0 partials of the z pass (nullable)
   int Gz;                      // number of dc_part entries
   const double* scale_stats;   // sums scaled by scale_stats[1]^2 (fused contrast), nullable

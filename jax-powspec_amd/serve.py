@@ -166,7 +166,8 @@ class _PlainEngine:
             _lib.check("pk_estimate", _lib.load().pk_estimate(
                 self.plan.handle, _device.ptr(j.xs), j.sx, _device.ptr(j.ys), j.sy, _device.ptr(j.zs), j.sz,
                 _device.ptr(j.ws), j.sw, j.np, *j.mins, j.box, _lib.STENCIL[j.mas], j.wrap, None,
-                _device.ptr(j.thr), _device.ptr(j.jst), j.nb, j.p, j.flags, j.shot, _device.ptr(o.sums),
+                _device.ptr(j.tables.thr), _device.ptr(j.tables.jst), _device.ptr(j.tables.modes),
+                _device.ptr(j.tables.ksum), j.nb, j.p, j.flags, j.shot, _device.ptr(o.sums),
                 _device.ptr(o.counts), _device.ptr(o.pk), _device.ptr(o.nmodes), _device.ptr(o.kmean),
                 _device.ptr(self.stats_t), _device.ptr(self.ws), self.ws_bytes, st.cuda_stream))
         ev = torch.cuda.Event()

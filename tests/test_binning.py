@@ -90,3 +90,33 @@ def test_decreasing_edges_rejected():
         binning.thresholds_from_grid_edges(np.array([1.0, 3.0, 2.0]), 16)
     with pytest.raises(ValueError):
         binning.thresholds_from_grid_edges(np.array([1.0]), 16)
+
+
+@pytest.mark.parametrize("n", [8, 12, 24, 64])
+@pytest.mark.parametrize("case", sorted(EDGE_CASES))
+def test_mode_tables_equal_brute_force_counts(n, case):
+    """Exact per-bin mode counts and sum of |k| from the m-multiplicity table equal a
+    brute-force pass over every stored half-spectrum entry under jnp.histogram's rule."""
+    edges = EDGE_CASES[case](n)
+    thr, _ = binning.thresholds_from_grid_edges(edges, n)
+    modes, ksum = binning.bin_modes(thr, n)
+    kx, ky, kz = R.half_spectrum_grids(n)
+    m = np.broadcast_to(kx * kx + ky * ky + kz * kz, (n, n, n // 2 + 1)).reshape(-1)
+    k = np.sqrt(m.astype(np.float64))
+    slot = R.jnp_histogram_slots(k, edges)
+    nb = edges.size - 1
+    sel = slot >= 0
+    want_c = np.bincount(slot[sel], minlength=nb)[:nb]
+    want_k = np.bincount(slot[sel], weights=k[sel], minlength=nb)[:nb]
+    assert modes.dtype == np.int64 and modes.shape == (nb,)
+    np.testing.assert_array_equal(modes, want_c)
+    np.testing.assert_allclose(ksum, want_k, rtol=1e-13, atol=0)
+    assert modes.sum() == sel.sum()
+
+
+def test_mode_multiplicity_totals():
+    # every stored entry counted once: n * n * (n/2 + 1) in total; m = 0 only at k = 0
+    for n in (8, 10, 64, 512):
+        mult = binning.mode_multiplicity(n)
+        assert mult.sum() == n * n * (n // 2 + 1)
+        assert mult[0] == 1 and mult[1] == 5     # k = 0; the six unit vectors less kz = -1
