@@ -616,11 +616,10 @@ int batch_launch(pk_batch* b, hipStream_t s) {
   if (e) return e;
   e = mass_finalize_jobs(b->d_jobs, nj, s);
   if (e) return e;
-  const int gy = (ypass_grid(n) + nj - 1) / nj;
-  e = fft_ypass_jobs(n, b->d_jobs, nj, gy, tw, s);
+  e = fft_ypass_jobs(n, b->d_jobs, nj, ypass_jobs_grid(n, nj), tw, s);
   if (e) return e;
-  int gx = (b->gmax + nj - 1) / nj;
-  if (gx < 1) gx = 1;
+  int gx = bin_jobs_grid(n, b->dims.max_nb, nj);
+  if (gx > b->gmax) gx = b->gmax;
   e = fft_xpass_bin_jobs(n, b->d_jobs, nj, b->dims.max_nb, gx, tw, s);
   if (e) return e;
   return bin_finalize_jobs(b->d_jobs, nj, b->dims.max_nb, gx, s);

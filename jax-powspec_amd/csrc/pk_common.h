@@ -124,6 +124,24 @@ __host__ __device__ constexpr bool fft_size_ok(int N) {
 }
 
 // ------------------------------------------------------------------------------------
+// twiddle sources: a full table e^{-2 pi i q / M} (LDS or global), or its first quarter
+// q < M / 4 (in LDS) with w^(a + b M/4) = w^a (-i)^b, exact up to the table's rounding
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 tw_at(const float2* tw, int q) { return tw[q]; }
+
+template <int M> struct TwQuarter {
+  const float2* p;
+};
+template <int M> __device__ __forceinline__ float2 tw_at(TwQuarter<M> t, int q) {
+  constexpr int Q = M / 4;
+  static_assert(M % 4 == 0, "quarter tables need M divisible by 4");
+  const int b = q / Q;
+  const float2 v = t.p[q - b * Q];
+  const float2 r = (b & 1) ? make_float2(v.y, -v.x) : v;
+  return (b & 2) ? make_float2(-r.x, -r.y) : r;
+}
+
+// ------------------------------------------------------------------------------------
 // One Stockham stage over C independent length-N transforms held in LDS.
 // Element (point p, column c) lives at buf[p * PS + c * CS].  The stage reads radix-R
 // butterflies at stride N/R, applies twiddles w^(r k) with k = j mod NS, runs the
@@ -132,8 +150,8 @@ __host__ __device__ constexpr bool fft_size_ok(int N) {
 // tw: table of e^{-2 pi i q / (N * TWS)}, q < N * TWS (stride TWS selects length N).
 // COLFAST: consecutive threads take consecutive columns (else consecutive butterflies).
 // ------------------------------------------------------------------------------------
-template <int N, int R, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS>
-__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
+template <int N, int R, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS, class TW>
+__device__ __forceinline__ void fft_stage(float2* buf, TW tw) {
   constexpr int B = N / R;
   constexpr int TASKS = B * C;
   constexpr int PER = (TASKS + NT - 1) / NT;
@@ -159,7 +177,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
       const int k = j % NS;
       if constexpr (NS > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], tw[(r * k * (N / (NS * R))) * TWS]);
+        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], tw_at(tw, (r * k * (N / (NS * R))) * TWS));
       }
       DFT<R>::run(v[t]);
       const int base = (j / NS) * NS * R + k;
@@ -172,7 +190,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* tw) {
 
 template <int N, int S, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS>
 struct FFTStages {
-  static __device__ __forceinline__ void run(float2* buf, const float2* tw) {
+  template <class TW> static __device__ __forceinline__ void run(float2* buf, TW tw) {
     constexpr int R = radix_at(N, S);
     if constexpr (R > 1) {
       fft_stage<N, R, NS, C, PS, CS, NT, COLFAST, TWS>(buf, tw);
@@ -182,10 +200,30 @@ struct FFTStages {
 };
 
 // forward length-N transforms of C columns in LDS (see fft_stage for the layout)
-template <int N, int C, int PS, int CS, int NT, bool COLFAST, int TWS>
-__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw) {
+template <int N, int C, int PS, int CS, int NT, bool COLFAST, int TWS, class TW>
+__device__ __forceinline__ void lds_fft(float2* buf, TW tw) {
   static_assert(fft_size_ok(N), "unsupported FFT length");
   FFTStages<N, 0, 1, C, PS, CS, NT, COLFAST, TWS>::run(buf, tw);
+}
+
+// Copies an R x W block of complex values (global row stride gs, LDS row stride PS) into
+// LDS with every global load issued before the first LDS store, so each wave keeps all of
+// its loads in flight instead of waiting on them one at a time.
+template <int R, int W, int PS, int NT>
+__device__ __forceinline__ void stage_tile(float2* __restrict__ buf, const float2* __restrict__ src, size_t gs) {
+  constexpr int TOT = R * W, PER = (TOT + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  float2 v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = tid + k * NT;
+    if (TOT % NT == 0 || i < TOT) v[k] = src[(size_t)(i / W) * gs + (i % W)];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = tid + k * NT;
+    if (TOT % NT == 0 || i < TOT) buf[(i / W) * PS + (i % W)] = v[k];
+  }
 }
 
 // ------------------------------------------------------------------------------------

@@ -57,12 +57,25 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
   for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
     const int x = tile / YT, yt = tile % YT;
     const float2* src = reinterpret_cast<const float2*>(mesh + ((size_t)x * N + (size_t)yt * C) * N);
-    for (int i = tid; i < C * NC; i += NT) {
-      float2 v = src[i];
-      v.x = v.x * s + b;
-      v.y = v.y * s + b;
-      dc += (double)v.x + (double)v.y;
-      buf[(i / NC) * CS + (i % NC)] = v;
+    {
+      // all loads in flight first; the FP64 k = 0 sum keeps its per-thread order
+      constexpr int TOT = C * NC, PER = (TOT + NT - 1) / NT;
+      float2 v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) v[k] = src[i];
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) {
+          v[k].x = v[k].x * s + b;
+          v[k].y = v[k].y * s + b;
+          dc += (double)v[k].x + (double)v[k].y;
+          buf[(i / NC) * CS + (i % NC)] = v[k];
+        }
+      }
     }
     __syncthreads();
     lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
@@ -163,7 +176,7 @@ k_xpass(float2* __restrict__ spec, const float2* __restrict__ twg) {
   for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
     const int kz = tile / YT, yt = tile % YT;
     float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
-    for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
+    stage_tile<N, C, PS, NT>(buf, base, N);
     __syncthreads();
     lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
     for (int i = tid; i < N * C; i += NT) base[(size_t)(i / C) * N + (i % C)] = buf[(i / C) * PS + (i % C)];
@@ -174,11 +187,12 @@ k_xpass(float2* __restrict__ spec, const float2* __restrict__ twg) {
 // ------------------------------------------------------------------------------------
 // bin lookup: slot of m = kx^2+ky^2+kz^2 under integer thresholds (see pk_api.h)
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ int find_bin(int m, const int* __restrict__ thr, const int* __restrict__ jst, int nb) {
+template <class JT>
+__device__ __forceinline__ int find_bin(int m, const int* __restrict__ thr, const JT* __restrict__ jst, int nb) {
   int j = (int)sqrtf((float)m);
   if (j * j > m) --j;
   else if ((j + 1) * (j + 1) <= m) ++j;
-  int b = jst[j];
+  int b = (int)jst[j];
   while (b + 1 < nb && thr[b + 1] <= m) ++b;
   if (b < 0 || m >= thr[nb]) return -1;
   return b;
@@ -247,7 +261,8 @@ constexpr int NACC = 3;       // S0, S2, S4 (mode counts and sums of |k| come fr
 template <int N> struct XBinLds {
   static constexpr size_t TILE = (size_t)N * (FFTCfg<N>::CX + 1) * 8;
   static constexpr int JLEN = isqrt_c(3 * (N / 2) * (N / 2)) + 2;   // isqrt-start table entries
-  static constexpr size_t HIST = (size_t)(HB_LDS + 1) * 4 + 3 * HB_LDS * 8 + (size_t)(N / 2 + 1) * 4 + (size_t)JLEN * 4;
+  static constexpr size_t HIST =
+      (size_t)(HB_LDS + 1) * 4 + 3 * HB_LDS * 8 + (size_t)(N / 2 + 1) * 4 + (size_t)JLEN * 2 + (size_t)(N / 4) * 8;
   // LDS-limited waves per SIMD (the histogram in HBM: no limit is imposed)
   template <bool LH> static constexpr int wpe() {
     return LH ? cmax(1, cmin(4, (int)((160 * 1024) / (TILE + HIST)) * (FFTCfg<N>::NTX / 64) / 4)) : 1;
@@ -266,29 +281,33 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   constexpr int YT = N / C;
   constexpr int NTILES = (N / 2 + 1) * YT;
   constexpr int H = N / 2;
-  // LDS: the tile, the S_0,2,4 histogram, the thresholds, the window factors of |k| <= N/2
-  // and the isqrt-start table (<= 80 KB at N = 512 so two blocks share a CU); twiddles are
-  // read through the caches
+  // LDS: the tile, the S_0,2,4 histogram, the thresholds, the window factors of |k| <= N/2,
+  // the isqrt-start table and the first quarter of the twiddles (<= 80 KB at N = 512 so two
+  // blocks share a CU)
   constexpr int JLEN = XBinLds<N>::JLEN;
   __shared__ float2 buf[N * PS];
   __shared__ int thr_s[LH ? HB_LDS + 1 : 1];
   __shared__ double hs[LH ? 3 * HB_LDS : 1];
   __shared__ float wt_s[LH ? H + 1 : 1];
-  __shared__ int jst_s[LH ? JLEN : 1];
-  const float2* tw = twg;
+  __shared__ short jst_s[LH ? JLEN : 1];   // bins < HB_LDS
+  __shared__ float2 twq[N / 4];
+  const TwQuarter<N> tw{twq};
   const int tid = threadIdx.x;
+  for (int i = tid; i < N / 4; i += NT) twq[i] = twg[i];
   double* gpart = j.partials + (size_t)bx * nb * NACC;
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
     for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0.0;
     for (int i = tid; i <= H; i += NT) wt_s[i] = wtab[i];
-    for (int i = tid; i < JLEN; i += NT) jst_s[i] = jst_g[i];
+    for (int i = tid; i < JLEN; i += NT) jst_s[i] = (short)jst_g[i];
   } else {
     for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;
   }
   __syncthreads();
-  const int* thr = LH ? thr_s : thr_g;
-  const int* jst = LH ? jst_s : jst_g;
+  auto bin_of = [&](int m) -> int {
+    if constexpr (LH) return find_bin(m, thr_s, jst_s, nb);
+    else return find_bin(m, thr_g, jst_g, nb);
+  };
   // window factor of array index i (symmetric: index N - i has the same |k|)
   auto wt = [&](int i) -> float {
     if constexpr (LH) return wt_s[i <= H ? i : N - i];
@@ -327,7 +346,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     const int r2 = ky * ky + kz2;
     // bins and window factors do not depend on the transform: the first iteration's table
     // reads are issued before the tile's load and FFT, each later one an iteration ahead
-    int slot_next = q0 <= H ? find_bin(q0 * q0 + r2, thr, jst, nb) : -1;
+    int slot_next = q0 <= H ? bin_of(q0 * q0 + r2) : -1;
     float wq_next = q0 <= H ? wt(q0) : 0.0f;
     const float wyz = wt(y) * wt(kz);
     if constexpr (XPF) {
@@ -337,8 +356,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
         if (i < N * C) buf[(i / C) * PS + (i % C)] = xpre[k];
       }
     } else {
-      const float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
-      for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
+      stage_tile<N, C, PS, NT>(buf, spec + (size_t)kz * N * N + (size_t)yt * C, N);
     }
     __syncthreads();
     if constexpr (XPF) {
@@ -351,7 +369,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       const int slot = slot_next;
       const float wqi = wq_next;
       const int qn = q + QSTEP;
-      slot_next = qn <= H ? find_bin(qn * qn + r2, thr, jst, nb) : -1;
+      slot_next = qn <= H ? bin_of(qn * qn + r2) : -1;
       wq_next = qn <= H ? wt(qn) : 0.0f;
       float p2 = 0.0f, v2 = 0.0f, v4 = 0.0f;
       if (slot >= 0) {
@@ -514,8 +532,7 @@ k_xpass_adjoint(Job j, const float2* __restrict__ twg, const float* __restrict__
   const int tid = threadIdx.x;
   for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
     const int kz = tile / YT, yt = tile % YT;
-    const float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
-    for (int i = tid; i < N * C; i += NT) buf[(i / C) * PS + (i % C)] = base[(size_t)(i / C) * N + (i % C)];
+    stage_tile<N, C, PS, NT>(buf, spec + (size_t)kz * N * N + (size_t)yt * C, N);
     __syncthreads();
     lds_fft<N, C, PS, 1, NT, true, 1>(buf, twg);
     const float wz = wt[kz];
@@ -576,9 +593,19 @@ k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, fl
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
     const int x = tile / YT, yt = tile % YT;
-    for (int i = tid; i < (NC + 1) * C; i += NT) {
-      const int c = i % C, k = i / C;
-      buf[c * CS + k] = bspec[((size_t)k * N + x) * N + (size_t)yt * C + c];
+    {
+      constexpr int TOT = (NC + 1) * C, PER = (TOT + NT - 1) / NT;
+      float2 v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) v[k] = bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)];
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) buf[(i % C) * CS + i / C] = v[k];
+      }
     }
     __syncthreads();
     // pairs (k, NC - k), k = 0..NC/2: each thread reads and rewrites only its own pair
@@ -674,6 +701,26 @@ template <int N> struct FFTLaunch {
     StageTimer t(ST_XPASS_PLAIN, s);
     hipLaunchKernelGGL(k_xpass<N>, dim3(grid), dim3(Cf::NTX), 0, s, spec, tw);
     return check_hip(hipGetLastError(), "x pass launch");
+  }
+  // blocks per job of a batch of njobs: the resident capacity of the whole chip shared out,
+  // at most one block per tile
+  static int ypass_jobs_grid(int njobs) {
+    static int cap = 0;
+    if (!cap) cap = persistent_grid(k_ypass_jobs<N>, Cf::NTY, 1 << 30);
+    return share(cap, njobs, YTILES);
+  }
+  static int bin_jobs_grid(bool lh, int njobs) {
+    static int c_l = 0, c_g = 0;
+    int& cap = lh ? c_l : c_g;
+    if (!cap)
+      cap = lh ? persistent_grid(k_xpass_bin_jobs<N, true>, Cf::NTX, 1 << 30)
+               : persistent_grid(k_xpass_bin_jobs<N, false>, Cf::NTX, 1 << 30);
+    return share(cap, njobs, bin_grid(lh));
+  }
+  static int share(int cap, int njobs, int limit) {
+    int g = (cap + njobs - 1) / (njobs < 1 ? 1 : njobs);
+    if (g > limit) g = limit;
+    return g < 1 ? 1 : g;
   }
   static int bin_grid(bool lh) {
     static int g_l = 0, g_g = 0;
@@ -778,6 +825,24 @@ int fft_xpass(int n, float2* spec, const float2* tw, hipStream_t s) {
 #undef PK_CASE
   }
   return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int ypass_jobs_grid(int n, int njobs) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::ypass_jobs_grid(njobs);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return 1;
+}
+
+int bin_jobs_grid(int n, int nb, int njobs) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::bin_jobs_grid(nb <= HB_LDS, njobs);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return 1;
 }
 
 int bin_grid(int n, int nb) {

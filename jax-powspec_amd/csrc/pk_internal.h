@@ -90,6 +90,9 @@ int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const do
               hipStream_t s);
 int fft_xpass(int n, float2* spec, const float2* tw, hipStream_t s);
 int bin_grid(int n, int nb);   // x-pass blocks of one estimate
+// blocks per job of a batch (y pass; x pass + binning, at most bin_grid(n, nb))
+int ypass_jobs_grid(int n, int njobs);
+int bin_jobs_grid(int n, int nb, int njobs);
 // single job (by value) or a device table of njobs (then gy / gx = blocks per job)
 int fft_ypass(int n, const Job& j, const float2* tw, hipStream_t s);
 int fft_xpass_bin(int n, const Job& j, const float2* tw, int grid, hipStream_t s);
