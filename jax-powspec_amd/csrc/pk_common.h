@@ -241,6 +241,29 @@ __host__ __device__ constexpr int isqrt_c(int v) {
 __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
+// Fixed-order block sum for NT a multiple of 64: xor-butterfly sums inside each wave, then
+// wave 0 combines the per-wave totals the same way (two barriers instead of a log2(NT)
+// tree).  red needs NT / 64 + 1 entries.  Every thread must call it; the result is in all
+// threads, and every thread has passed both barriers before any reads it.
+template <int NT>
+__device__ __forceinline__ double block_sum_f64_waves(double v, double* red) {
+  static_assert(NT % 64 == 0, "whole waves only");
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    double t = threadIdx.x < NW ? red[threadIdx.x] : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      if (off < NW * 2) t += __shfl_xor(t, off, 64);
+    if (threadIdx.x == 0) red[NW] = t;
+  }
+  __syncthreads();
+  return red[NW];
+}
+
 __host__ __device__ constexpr int pow2_ceil(int v) {
   int p = 1;
   while (p < v) p <<= 1;

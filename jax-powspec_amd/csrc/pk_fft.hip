@@ -30,13 +30,18 @@ template <int N> struct FFTCfg {
   // x pass: CX columns (consecutive ky) per tile
   static constexpr int CX = cmin(pow2_div(N), cmin(64, cmax(N >= 1024 ? 8 : 16, pow2_floor(4096 / N))));
   static constexpr int NTX = cmin(1024, cmax(64, (CX * N / 16) / 64 * 64));
+  // z passes: waves per SIMD their LDS allows (tile, twiddles, FP64 reduction; capped at
+  // 4), to which their register budget is held
+  static constexpr int WPEZ = cmax(
+      1, cmin(4, (int)((160 * 1024) / ((size_t)CZ * (NC + 1) * 8 + (size_t)N * 8 + (size_t)NTZ * 8 + 1024)) *
+                     (NTZ / 64) / 4));
 };
 
 // ------------------------------------------------------------------------------------
 // z pass: real rows -> half spectrum rows, written transposed to [kz][x][y]
 // ------------------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(FFTCfg<N>::NTZ)
+__global__ void __launch_bounds__(FFTCfg<N>::NTZ) __attribute__((amdgpu_waves_per_eu(FFTCfg<N>::WPEZ)))
 k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2* __restrict__ twg,
         const double* __restrict__ stats, double* __restrict__ dc_part) {
   using Cf = FFTCfg<N>;
@@ -78,7 +83,7 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
       }
     }
     __syncthreads();
-    lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+    lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
     // split: X[k] = Fe + e^{-2 pi i k / N} Fo, Fe = (Z_k + Z*_{NC-k}) / 2, Fo = -i (Z_k - Z*_{NC-k}) / 2
     for (int i = tid; i < (NC + 1) * C; i += NT) {
       const int c = i % C, k = i / C;
@@ -634,7 +639,7 @@ k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, fl
       if (kk != k && kk < NC) row[kk] = make_float2(zkk.x, -zkk.y);
     }
     __syncthreads();
-    lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+    lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
     float2* orow = reinterpret_cast<float2*>(out + ((size_t)x * N + (size_t)yt * C) * N);
     for (int i = tid; i < C * NC; i += NT) {
       const int c = i / NC, jj = i % NC;

@@ -36,22 +36,30 @@ template <int N, int TY_> struct FusedCfg {
   static constexpr size_t FFT_LDS = (size_t)TY * CS * sizeof(float2);
   static constexpr size_t LDS = FIXED ? cmax((int)((size_t)TY * N * 8), (int)FFT_LDS) : FFT_LDS;
   static constexpr int PER = (TY * N + NT - 1) / NT;     // cells per thread in the conversion
+  // waves per SIMD the LDS allows (capped at 4): the register budget is held to it
+  static constexpr int WPE =
+      cmax(1, cmin(4, (int)((160 * 1024) / (LDS + 8192 + (size_t)NT * 8)) * (NT / 64) / 4));
 };
 
-// Bucket descriptors of one tile in LDS (see gather_buckets): keys, first record and
-// exclusive prefix of sizes.  Block-cooperative; ends with a barrier.
+// Bucket descriptors of one tile in LDS: first record and exclusive prefix of sizes.
 struct TileBuckets {
-  int key[MAX_TILE_BUCKETS];
   int base[MAX_TILE_BUCKETS];
   int pre[MAX_TILE_BUCKETS + 1];
   int nbk;
 };
 
+// Block sum of the tile mass: wave butterflies for whole-wave blocks, else the tree.
+template <int NT>
+__device__ __forceinline__ double tile_sum(double v, double* red) {
+  if constexpr (NT % 64 == 0) return block_sum_f64_waves<NT>(v, red);
+  else return block_sum_f64(v, red, NT);
+}
+
 template <int N, int TY>
 __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const float2* __restrict__ twg) {
   using Cf = FusedCfg<N, TY>;
   constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
-  constexpr int U = NT >= 1024 ? 2 : 4;   // records per thread prefetched for the next tile
+  constexpr int U = NT >= 1024 ? 2 : 4;   // records per thread loaded ahead
   extern __shared__ float2 buf[];   // FFT rows: C rows of CS complex = 2 * CS floats per row
   __shared__ float2 tw[N];
   __shared__ TileBuckets tb[2];
@@ -72,17 +80,49 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   const double inv_scale = j.aux[1];
   const float c0 = (float)j.stats[3];
   float2* spec = j.spec;
+  const int* __restrict__ key_start = j.key_start;
+  const int nbk = tile_bucket_count(g, N, tsc, irregular);   // the same for every tile
+  // Bucket bounds of tile t: lane q < nbk loads its own key's [start, end) and does not
+  // wait, so the loads of the next tile's bounds land while this tile deposits.
+  auto issue = [&](int t, int& lo, int& hi) {
+    if (tid < nbk) {
+      const int key = tile_bucket_key(g, N, tsc, t / g.nyt, t % g.nyt, tid);
+      lo = key_start[key];
+      hi = key_start[key + 1];
+    }
+  };
+  // ... then wave 0 scans the sizes into B (one barrier, which also ends the deposit)
+  auto publish = [&](TileBuckets& B, int lo, int hi) {
+    if (tid < 64) {
+      int incl = tid < nbk ? hi - lo : 0;
+#pragma unroll
+      for (int off = 1; off < MAX_TILE_BUCKETS; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (tid >= off) incl += v;
+      }
+      if (tid < nbk) {
+        B.base[tid] = lo;
+        B.pre[tid + 1] = incl;
+      }
+      if (tid == 0) {
+        B.pre[0] = 0;
+        B.nbk = nbk;
+      }
+    }
+    __syncthreads();
+  };
   // record idx (< total) of the tile buckets B; a thread visits increasing idx, so its
   // bucket cursor b only moves forward
   auto record = [&](const TileBuckets& B, int idx, int& b) -> float4 {
     while (b + 1 < B.nbk && B.pre[b + 1] <= idx) ++b;
     return load_record(sorted, (size_t)B.base[b] + (idx - B.pre[b]), has_w);
   };
-  // Software pipeline over this block's tiles: the next tile's bucket descriptors are set
-  // up after the current deposit, and its first U * NT records are loaded into registers
-  // before the current tile's FFT, so their latency overlaps the transform and the writes.
-  gather_buckets(g, N, tsc, irregular, bx / g.nyt, bx % g.nyt, j.key_start, tb[0].key, tb[0].base, tb[0].pre,
-                 &tb[0].nbk);
+  // Software pipeline over this block's tiles: the next tile's bucket bounds load during the
+  // current deposit, and its first U * NT records are loaded into registers before the
+  // current tile's FFT, so their latency overlaps the transform and the writes.
+  int lo = 0, hi = 0;
+  issue(bx, lo, hi);
+  publish(tb[0], lo, hi);
   float4 rp[U];
   auto prefetch = [&](const TileBuckets& B) {
     const int total = B.pre[B.nbk];
@@ -98,7 +138,9 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   for (int t = bx; t < ntiles; t += nbx) {
     const int xt = t / g.nyt, yt = t % g.nyt;
     const int y0 = yt * C;
+    const int tn = t + nbx;
     const TileBuckets& B = tb[cur];
+    if (tn < ntiles) issue(tn, lo, hi);
     if constexpr (Cf::FIXED) {
       for (int i = tid; i < C * N; i += NT) acc[i] = 0ull;
     } else {
@@ -115,25 +157,23 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       });
     };
     const int total = B.pre[B.nbk];
+    int bc = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (tid + u * NT < total) dep(rp[u]);
-    int bc = 0;
     for (int idx0 = tid + U * NT; idx0 < total; idx0 += U * NT) {
-      float4 r[U];
+      float4 c[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int idx = idx0 + u * NT;
-        if (idx < total) r[u] = record(B, idx, bc);
+        if (idx < total) c[u] = record(B, idx, bc);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (idx0 + u * NT < total) dep(r[u]);
+        if (idx0 + u * NT < total) dep(c[u]);
     }
-    const int tn = t + nbx;
-    if (tn < ntiles)   // block-uniform; its barriers also close the deposit
-      gather_buckets(g, N, tsc, irregular, tn / g.nyt, tn % g.nyt, j.key_start, tb[cur ^ 1].key, tb[cur ^ 1].base,
-                     tb[cur ^ 1].pre, &tb[cur ^ 1].nbk);
+    if (tn < ntiles)   // block-uniform; the barrier also closes the deposit
+      publish(tb[cur ^ 1], lo, hi);
     else
       __syncthreads();
     // tile mass (FP64) and rho - c into the FFT rows
@@ -164,11 +204,12 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
         *p = v - c0;
       }
     }
-    const double m = block_sum_f64(msum, red, NT);   // ends with a barrier
+    const double m = tile_sum<NT>(msum, red);   // ends with a barrier
     if (tid == 0) j.tile_mass[t] = m;
     if (tn < ntiles) prefetch(tb[cur ^ 1]);
-    // z transform of the C rows (pairs of reals as NC complex, then the split)
-    lds_fft<NC, C, 1, CS, NT, false, 2>(buf, tw);
+    // z transform of the C rows (pairs of reals as NC complex, then the split); consecutive
+    // lanes take consecutive rows, so the butterflies' LDS accesses are free of conflicts
+    lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
     for (int i = tid; i < (NC + 1) * C; i += NT) {
       const int c = i % C, k = i / C;
       const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
@@ -185,12 +226,13 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
 }
 
 template <int N, int TY>
-__global__ void __launch_bounds__((FusedCfg<N, TY>::NT)) k_fused_deposit_zpass(Job j, const float2* __restrict__ tw) {
+__global__ void __launch_bounds__((FusedCfg<N, TY>::NT)) __attribute__((amdgpu_waves_per_eu(FusedCfg<N, TY>::WPE)))
+k_fused_deposit_zpass(Job j, const float2* __restrict__ tw) {
   fused_body<N, TY>(j, blockIdx.x, gridDim.x, tw);
 }
 
 template <int N, int TY>
-__global__ void __launch_bounds__((FusedCfg<N, TY>::NT))
+__global__ void __launch_bounds__((FusedCfg<N, TY>::NT)) __attribute__((amdgpu_waves_per_eu(FusedCfg<N, TY>::WPE)))
 k_fused_deposit_zpass_jobs(const Job* __restrict__ jobs, const float2* __restrict__ tw) {
   const Job& j = jobs[blockIdx.y];
   if (!j.active) return;

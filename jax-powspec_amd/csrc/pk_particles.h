@@ -258,7 +258,7 @@ __device__ __forceinline__ int row_tile_key(const ScatterArgs& a, const TileGeom
 }
 
 // Buckets whose particles can deposit into tile (xt, yt): filled into key[] (<= 16).
-__device__ __forceinline__ int row_tile_buckets(const TileGeom& g, int n, bool tsc, int xt, int yt, int* keys) {
+__host__ __device__ __forceinline__ int row_tile_buckets(const TileGeom& g, int n, bool tsc, int xt, int yt, int* keys) {
   int xs[3], nxs = 0;
   xs[nxs++] = xt;
   if (n > 1) xs[nxs++] = (xt + n - 1) % n;                 // lower x neighbour (all of it)
@@ -284,6 +284,39 @@ __device__ __forceinline__ int row_tile_buckets(const TileGeom& g, int n, bool t
   for (int i = 0; i < nxs; ++i)
     for (int j = 0; j < ny; ++j) keys[nk++] = (xs[i] * g.nyt + yc[j]) * g.ncls + ycl[j];
   return nk;
+}
+
+// Number of buckets row_tile_buckets lists (the same for every tile), plus the irregular one.
+__host__ __device__ __forceinline__ int tile_bucket_count(const TileGeom& g, int n, bool tsc, bool irregular) {
+  const int nxs = 1 + (n > 1 ? 1 : 0) + (tsc && n > 2 ? 1 : 0);
+  const int ny = g.ncls + (g.nyt >= 2 ? (tsc ? 2 : 1) : 0);
+  return nxs * ny + (irregular ? 1 : 0);
+}
+
+// Key q of tile (xt, yt) in row_tile_buckets' order; q = nxs * ny is the irregular bucket.
+// Each lane computes its own key, so no lane waits for another before loading key starts.
+__host__ __device__ __forceinline__ int tile_bucket_key(const TileGeom& g, int n, bool tsc, int xt, int yt, int q) {
+  const int nxs = 1 + (n > 1 ? 1 : 0) + (tsc && n > 2 ? 1 : 0);
+  const int ny = g.ncls + (g.nyt >= 2 ? (tsc ? 2 : 1) : 0);
+  if (q >= nxs * ny) return g.nkeys - 1;
+  const int i = q / ny, jj = q - (q / ny) * ny;
+  const int x = i == 0 ? xt : (i == 1 ? (xt + n - 1) % n : (xt + 1) % n);
+  int yc = yt, ycl = jj;
+  if (jj >= g.ncls) {
+    const int e = jj - g.ncls;
+    const int lo = (yt + g.nyt - 1) % g.nyt, hi = (yt + 1) % g.nyt;
+    if (tsc && g.nyt >= 3) {
+      yc = e == 0 ? lo : hi;
+      ycl = e == 0 ? 2 : 1;
+    } else if (tsc) {
+      yc = lo;
+      ycl = e == 0 ? 1 : 2;
+    } else {
+      yc = lo;
+      ycl = 1;
+    }
+  }
+  return (x * g.nyt + yc) * g.ncls + ycl;
 }
 
 // Block-cooperative setup of the buckets of tile (xt, yt) (plus the irregular bucket when
