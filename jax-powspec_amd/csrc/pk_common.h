@@ -150,8 +150,11 @@ template <int M> __device__ __forceinline__ float2 tw_at(TwQuarter<M> t, int q) 
 // tw: table of e^{-2 pi i q / (N * TWS)}, q < N * TWS (stride TWS selects length N).
 // COLFAST: consecutive threads take consecutive columns (else consecutive butterflies).
 // ------------------------------------------------------------------------------------
-template <int N, int R, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS, class TW>
+template <int N, int R, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS, int SK, class TW>
 __device__ __forceinline__ void fft_stage(float2* buf, TW tw) {
+  // point p of column c: p * PS + c * CS, plus p >> SK with a skew (SK > 0), which spreads
+  // the stride-R autosort writes over the LDS banks
+  auto at = [](int p, int c) { return p * PS + (SK > 0 ? (p >> SK) : 0) + c * CS; };
   constexpr int B = N / R;
   constexpr int TASKS = B * C;
   constexpr int PER = (TASKS + NT - 1) / NT;
@@ -164,7 +167,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, TW tw) {
       const int j = COLFAST ? task / C : task % B;
       const int c = COLFAST ? task % C : task / B;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[t][r] = buf[(j + r * B) * PS + c * CS];
+      for (int r = 0; r < R; ++r) v[t][r] = buf[at(j + r * B, c)];
     }
   }
   __syncthreads();
@@ -182,28 +185,29 @@ __device__ __forceinline__ void fft_stage(float2* buf, TW tw) {
       DFT<R>::run(v[t]);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(base + r * NS) * PS + c * CS] = v[t][r];
+      for (int r = 0; r < R; ++r) buf[at(base + r * NS, c)] = v[t][r];
     }
   }
   __syncthreads();
 }
 
-template <int N, int S, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS>
+template <int N, int S, int NS, int C, int PS, int CS, int NT, bool COLFAST, int TWS, int SK>
 struct FFTStages {
   template <class TW> static __device__ __forceinline__ void run(float2* buf, TW tw) {
     constexpr int R = radix_at(N, S);
     if constexpr (R > 1) {
-      fft_stage<N, R, NS, C, PS, CS, NT, COLFAST, TWS>(buf, tw);
-      FFTStages<N, S + 1, NS * R, C, PS, CS, NT, COLFAST, TWS>::run(buf, tw);
+      fft_stage<N, R, NS, C, PS, CS, NT, COLFAST, TWS, SK>(buf, tw);
+      FFTStages<N, S + 1, NS * R, C, PS, CS, NT, COLFAST, TWS, SK>::run(buf, tw);
     }
   }
 };
 
 // forward length-N transforms of C columns in LDS (see fft_stage for the layout)
-template <int N, int C, int PS, int CS, int NT, bool COLFAST, int TWS, class TW>
+template <int N, int C, int PS, int CS, int NT, bool COLFAST, int TWS, int SK = 0, class TW>
 __device__ __forceinline__ void lds_fft(float2* buf, TW tw) {
   static_assert(fft_size_ok(N), "unsupported FFT length");
-  FFTStages<N, 0, 1, C, PS, CS, NT, COLFAST, TWS>::run(buf, tw);
+  static_assert(SK == 0 || PS == 1, "skewed layouts take unit point stride");
+  FFTStages<N, 0, 1, C, PS, CS, NT, COLFAST, TWS, SK>::run(buf, tw);
 }
 
 // Copies an R x W block of complex values (global row stride gs, LDS row stride PS) into

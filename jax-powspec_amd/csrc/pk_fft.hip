@@ -111,13 +111,18 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
 template <int N, bool CONJ = false>
 __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const float2* __restrict__ twg, int bx, int nbx) {
   using Cf = FFTCfg<N>;
-  constexpr int C = Cf::CY, NT = Cf::NTY, CS = N + 1;
+  // rows are skewed in LDS (point p at p + p / 8), which keeps the butterflies' strided
+  // writes off each other's banks; twiddles from a quarter table
+  constexpr int SK = 3;
+  constexpr int C = Cf::CY, NT = Cf::NTY, CS = N + (SK > 0 ? (N >> SK) : 0) + 1;
   constexpr int NTILES = (N / 2 + 1) * N / C;
   __shared__ float2 buf[C * CS];
-  __shared__ float2 tw[N];
+  __shared__ float2 twq[N / 4];
+  const TwQuarter<N> tw{twq};
   constexpr int PER = (C * N + NT - 1) / NT;
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int i = tid; i < N / 4; i += NT) twq[i] = twg[i];
+  auto at = [](int p, int c) { return c * CS + p + (SK > 0 ? (p >> SK) : 0); };
   // the next tile's rows are loaded into registers while this tile is transformed
   float2 pre[PER];
   auto load = [&](int tile) {
@@ -136,14 +141,14 @@ __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const floa
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = tid + k * NT;
-      if (i < C * N) buf[(i / N) * CS + (i % N)] = pre[k];
+      if (i < C * N) buf[at(i % N, i / N)] = pre[k];
     }
     __syncthreads();
     if (tile + nbx < NTILES) load(tile + nbx);
-    lds_fft<N, C, 1, CS, NT, false, 1>(buf, tw);
+    lds_fft<N, C, 1, CS, NT, false, 1, SK>(buf, tw);
     float2* row = spec + (size_t)tile * C * N;
     for (int i = tid; i < C * N; i += NT) {
-      float2 v = buf[(i / N) * CS + (i % N)];
+      float2 v = buf[at(i % N, i / N)];
       if constexpr (CONJ) v.y = -v.y;
       row[i] = v;
     }
@@ -376,31 +381,23 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       const int qn = q + QSTEP;
       slot_next = qn <= H ? bin_of(qn * qn + r2) : -1;
       wq_next = qn <= H ? wt(qn) : 0.0f;
-      float p2 = 0.0f, v2 = 0.0f, v4 = 0.0f;
-      if (slot >= 0) {
-        const int m = q * q + r2;
-        const float2 a = buf[q * PS + c];
-        p2 = a.x * a.x + a.y * a.y;
-        if (q != 0 && q != H) {
-          const float2 bb = buf[(N - q) * PS + c];
-          p2 += bb.x * bb.x + bb.y * bb.y;
-        }
-        p2 *= wqi * wyz;
-        // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
-        //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
-        // so L2 is exactly 0 where 3 kz^2 = m.  The k = 0 mode's power is added by the
-        // finalize from the FP64 sum of the mesh (mu = 0 there, :37).
-        if (m > 0) {
-          const float rm = 1.0f / (float)m;
-          const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
-          const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
-          const float l4 = 0.125f * (float)num4 * rm * rm;
-          v2 = p2 * l2;
-          v4 = p2 * l4;
-        } else {
-          p2 = 0.0f;
-        }
-      }
+      // branch-free: idle lanes (q > H) read rows inside the tile (q < N) and are zeroed
+      // through slot < 0; the k = 0 mode's power is added by the finalize from the FP64
+      // sum of the mesh (mu = 0 there, :37)
+      const int m = q * q + r2;
+      const float2 a = buf[q * PS + c];
+      const float2 bb = buf[(q == 0 ? 0 : N - q) * PS + c];
+      const float pair = (q != 0 && q != H) ? 1.0f : 0.0f;
+      float p2 = (a.x * a.x + a.y * a.y) + pair * (bb.x * bb.x + bb.y * bb.y);
+      p2 = (slot >= 0 && m > 0) ? p2 * (wqi * wyz) : 0.0f;
+      // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
+      //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
+      // so L2 is exactly 0 where 3 kz^2 = m; 1 / m to within 1 ulp (hardware reciprocal)
+      const float rm = __builtin_amdgcn_rcpf((float)(m > 0 ? m : 1));
+      const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
+      const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
+      const float l4 = 0.125f * (float)num4 * rm * rm;
+      float v2 = p2 * l2, v4 = p2 * l4;
       // runs of equal slot in lane order -> one histogram update per run (its last lane)
       const bool tail = wave_run_sums(slot, p2, v2, v4);
       if (!tail || slot < 0) continue;
