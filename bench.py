@@ -106,8 +106,25 @@ STAGE_KERNELS = {
 
 def stage_kernel(stage, n, fused):
     if stage == "deposit":
-        return f"k_fused_deposit_zpass<{n}>" if fused else "k_rows_deposit"
+        # default row-bundle height of the fused kernel (largest of 32, 24, 16 dividing n)
+        ty = n if n <= 32 else (32 if n % 32 == 0 else (24 if n % 24 == 0 else 16))
+        return f"k_fused_deposit_zpass<{n}, {ty}>" if fused else "k_rows_deposit"
     return STAGE_KERNELS.get(stage, stage)
+
+
+def trace_summary(stats_csv, out_path, title, command):
+    """The library's kernels from a rocprofv3 --kernel-trace --stats kernel_stats.csv."""
+    import csv
+    rows = [r for r in csv.DictReader(open(stats_csv)) if r["Name"].startswith(("pk::", "void pk::"))]
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    lines = [f"# rocprofv3 kernel-trace statistics: {title}", f"# command: {command}",
+             "# kernel\tcalls\tavg_us\tmin_us\tmax_us\ttotal_%"]
+    for r in rows:
+        name = r["Name"].split("(")[0].replace("void ", "")
+        lines.append(f"{name}\t{r['Calls']}\t{float(r['AverageNs']) / 1e3:.1f}\t{float(r['MinNs']) / 1e3:.1f}\t"
+                     f"{float(r['MaxNs']) / 1e3:.1f}\t{float(r['Percentage']):.2f}")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def read_traffic(path):
@@ -430,15 +447,21 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mock batched comparison")
     ap.add_argument("--only-batched", action="store_true", help="run only the batched comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the serving benchmark")
+    ap.add_argument("--no-small", action="store_true", help="skip the 128^3 / 1e6-particle plain case")
     ap.add_argument("--only-served", action="store_true", help="run only the serving benchmark")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the calibrated HBM traffic summary of two rocprofv3 --pmc runs and exit")
     ap.add_argument("--pmc-command", default="", help="command line recorded in the --pmc-summary header")
+    ap.add_argument("--trace-summary", nargs=3, metavar=("STATS_CSV", "OUT", "TITLE"),
+                    help="write the library's kernels of a rocprofv3 kernel_stats.csv to OUT and exit")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_cic512_2e7.txt"),
                     help="traffic summary read for roofline.traffic")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.traffic, args.pmc_command)
+        return
+    if args.trace_summary:
+        trace_summary(*args.trace_summary, args.pmc_command)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the path's "
@@ -531,7 +554,7 @@ def main():
             f"({roof_path / HBM_PEAK_GBS:.3f} of peak)")
     batched = served = None
     del est
-    small = bench_small(dev, cpu=not args.no_cpu)
+    small = None if args.no_small else bench_small(dev, cpu=not args.no_cpu)
     log(f"small: {small}")
     if not args.no_batched:
         batched = bench_batched(dev)

@@ -30,6 +30,10 @@
  *     m = kx^2 + ky^2 + kz^2 (integer wavenumbers), except that thr[nb] is the first m above
  *     the closed last edge.  jstart: int32[jmax+1], jstart[j] = #{b < nb : thr[b] <= j*j} - 1.
  *     Both are produced on the host from the FP64 edges (jax_powspec_amd.binning).
+ *   - mode tables: int64 modes[nb] = number of stored half-spectrum entries in bin b
+ *     (kx, ky in [0, n), kz in [0, n/2], each once, k = 0 included when its bin exists) and
+ *     FP64 ksum[nb] = sum of |k| over them in units of k_F (needed with PK_FLAG_KMEAN only).
+ *     They depend on n and thr only and are exact (jax_powspec_amd.binning.bin_modes).
  */
 #ifndef PK_API_H
 #define PK_API_H
@@ -164,7 +168,7 @@ int pk_estimate_backward(pk_plan* plan,
                          const double* stats, const double* grad_pk, const int64_t* counts,
                          float* grad_pos, float* grad_w, void* workspace, size_t workspace_bytes, void* stream);
 
-/* --- batches: many catalogues/* --- batches: many catalogues of one mesh side per launch --------------------------
+/* --- batches: many catalogues of one mesh side per launch --------------------------
  * No reference counterpart: the reference estimates one catalogue per call.  A batch
  * runs up to max_jobs estimates (each as pk_estimate: its own particles, stencil, window,
  * bins, shot noise and outputs) with one launch per stage for all of them: blockIdx.y
