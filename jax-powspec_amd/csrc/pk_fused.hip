@@ -147,11 +147,13 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       for (int i = tid; i < C * 2 * CS; i += NT) rows[i] = 0.0f;
     }
     __syncthreads();
+    // fixed point: the record's weight carries the power-of-two scale, so each corner is
+    // (W w) S exactly as before and costs only the round and the int32 conversion
     auto dep = [&](const float4& r) {
-      for_each_corner_on_x(a, r.x, r.y, r.z, r.w, xt, [&](int iy, int iz, float wt) {
+      for_each_corner_on_x(a, r.x, r.y, r.z, Cf::FIXED ? r.w * scale : r.w, xt, [&](int iy, int iz, float wt) {
         const int ly = iy - y0;
         if ((unsigned)ly < (unsigned)C) {
-          if constexpr (Cf::FIXED) atomicAdd(&acc[ly * N + iz], to_fixed(wt, scale));
+          if constexpr (Cf::FIXED) atomicAdd(&acc[ly * N + iz], to_fixed(wt));
           else atomicAdd(&rows[ly * 2 * CS + iz], wt);
         }
       });

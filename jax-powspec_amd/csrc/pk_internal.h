@@ -40,7 +40,7 @@ struct TileGeom {
 // integer atomics run ~13x faster than float ones, and integer sums are order-independent
 constexpr size_t FIXED_LDS_MAX = 131072;
 // fixed-point scale (aux[0] = S, aux[1] = 1 / S) lives in the scatter workspace; deposits
-// are rint(weight * S) with S = 2^32 / 2^ceil(log2 max|w|), so |sum| < 2^63 for < 2^31 particles
+// are rint(weight * S) with S = 2^30 / 2^e, max|w| < 2^e, so |sum| < 2^61 for < 2^31 particles
 constexpr int AUX_LEN = 4;
 
 // One estimate's arguments and scratch as the shared kernels see them.  The plain entry

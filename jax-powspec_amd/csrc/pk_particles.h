@@ -345,10 +345,14 @@ __device__ __forceinline__ void gather_buckets(const TileGeom& g, int n, bool ts
   __syncthreads();
 }
 
-// 64-bit fixed-point deposit value (two's complement) of a float weight; scale = 2^k.
-__device__ __forceinline__ unsigned long long to_fixed(float wt, float scale) {
-  return (unsigned long long)__float2ll_rn(wt * scale);
+// 64-bit fixed-point deposit value (two's complement) of a weight already multiplied by
+// the scale S = 2^(30 - e) with 2^e > max|w|: every corner weight is <= 1, so |v| < 2^30,
+// rint(v) fits an int32 and the conversion is one round and one convert (a float -> int64
+// conversion would take ten instructions per corner).
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+  return (unsigned long long)(long long)(int)__builtin_rintf(v);
 }
+__device__ __forceinline__ unsigned long long to_fixed(float wt, float scale) { return to_fixed(wt * scale); }
 
 // Deposit every record of the tile's buckets (set up by gather_buckets) whose corners lie
 // on plane xt and rows [y0, y0 + ty): f(ly, iz, weight).  The bucket of a flat index is

@@ -213,12 +213,13 @@ __device__ __forceinline__ void scan_top_body(const Job& j) {
   if (threadIdx.x == 0) {
     const double n = (double)j.a.n;
     j.stats[3] = t / (n * n * n);
-    // fixed-point scale S = 2^32 / 2^ceil(log2 max|w|): every deposit |rint(w S)| <= 2^32
+    // fixed-point scale S = 2^30 / 2^e with max|w| < 2^e: every deposit |rint(w W S)| < 2^30
+    // for corner weights W <= 1, so it converts through int32
     int e = 0;
     const double wmax = __longlong_as_double((long long)wmax_bits);
     if (wmax > 0.0 && wmax < 1e300) (void)frexp(wmax, &e);
-    j.aux[0] = ldexp(1.0, 32 - e);
-    j.aux[1] = ldexp(1.0, e - 32);
+    j.aux[0] = ldexp(1.0, 30 - e);
+    j.aux[1] = ldexp(1.0, e - 30);
   }
 }
 __global__ void __launch_bounds__(SCAN_NT) k_scan_top(Job j) { scan_top_body(j); }
