@@ -14,6 +14,9 @@ constexpr float POS_LIMIT = 1073741824.0f;  // |pos| >= 2^30 or non-finite: part
 constexpr int MAX_TILE_BUCKETS = 16;          // buckets a row-bundle tile reads (incl. irregular)
 
 __device__ __forceinline__ int floor_mod(int a, int n) {
+  // indexes of positions inside the box (or one period above) take two instructions; the
+  // integer division only runs for lanes that need it
+  if ((unsigned)a < (unsigned)(2 * n)) return a >= n ? a - n : a;
   int r = a % n;
   return r < 0 ? r + n : r;
 }
@@ -110,11 +113,17 @@ __device__ __forceinline__ void for_each_corner(const ScatterArgs& a, float px, 
       w0[ax] = 1.0f - d;
       w1[ax] = d;
     }
+    // weights ((wx w) wy) wz: the plane and row factors are shared by their corners
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int sx = s >> 2, sy = (s >> 1) & 1, sz = s & 1;
-      f(sx ? c1[0] : c0[0], sy ? c1[1] : c0[1], sz ? c1[2] : c0[2],
-        (sx ? w1[0] : w0[0]) * (sy ? w1[1] : w0[1]) * (sz ? w1[2] : w0[2]) * w);
+    for (int sx = 0; sx < 2; ++sx) {
+      const float ax = (sx ? w1[0] : w0[0]) * w;
+#pragma unroll
+      for (int sy = 0; sy < 2; ++sy) {
+        const float axy = ax * (sy ? w1[1] : w0[1]);
+#pragma unroll
+        for (int sz = 0; sz < 2; ++sz)
+          f(sx ? c1[0] : c0[0], sy ? c1[1] : c0[1], sz ? c1[2] : c0[2], axy * (sz ? w1[2] : w0[2]));
+      }
     }
   } else {  // TSC
     const float pos[3] = {px, py, pz};
@@ -133,12 +142,17 @@ __device__ __forceinline__ void for_each_corner(const ScatterArgs& a, float px, 
       wt[ax][1] = 0.75f - d * d;
       wt[ax][2] = 0.5f * hp * hp;
     }
+    // weights ((wx w) wy) wz, as for CIC
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i) {
+      const float ax = wt[0][i] * w;
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
+      for (int j = 0; j < 3; ++j) {
+        const float axy = ax * wt[1][j];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) f(c[0][i], c[1][j], c[2][k], wt[0][i] * wt[1][j] * wt[2][k] * w);
+        for (int k = 0; k < 3; ++k) f(c[0][i], c[1][j], c[2][k], axy * wt[2][k]);
+      }
+    }
   }
 }
 
@@ -179,11 +193,12 @@ __device__ __forceinline__ void for_each_corner_on_x(const ScatterArgs& a, float
 #pragma unroll
     for (int sx = 0; sx < 2; ++sx) {
       if ((sx ? c1[0] : c0[0]) != xt) continue;
-      const float wx = sx ? w1[0] : w0[0];
+      const float ax = (sx ? w1[0] : w0[0]) * w;   // same products as for_each_corner
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int sy = s >> 1, sz = s & 1;
-        f(sy ? c1[1] : c0[1], sz ? c1[2] : c0[2], wx * (sy ? w1[1] : w0[1]) * (sz ? w1[2] : w0[2]) * w);
+      for (int sy = 0; sy < 2; ++sy) {
+        const float axy = ax * (sy ? w1[1] : w0[1]);
+#pragma unroll
+        for (int sz = 0; sz < 2; ++sz) f(sy ? c1[1] : c0[1], sz ? c1[2] : c0[2], axy * (sz ? w1[2] : w0[2]));
       }
     }
   } else {  // TSC
@@ -206,10 +221,13 @@ __device__ __forceinline__ void for_each_corner_on_x(const ScatterArgs& a, float
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       if (c[0][i] != xt) continue;
+      const float ax = wt[0][i] * w;   // same products as for_each_corner
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
+      for (int j = 0; j < 3; ++j) {
+        const float axy = ax * wt[1][j];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) f(c[1][j], c[2][k], wt[0][i] * wt[1][j] * wt[2][k] * w);
+        for (int k = 0; k < 3; ++k) f(c[1][j], c[2][k], axy * wt[2][k]);
+      }
     }
   }
 }
