@@ -323,72 +323,116 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     if constexpr (LH) return wt_s[i <= H ? i : N - i];
     else return wtab[i];
   };
-  // pair tasks: |kx| = q in [0, N/2], column c; entries q and N-q share m and mu.  Every
-  // thread keeps one column (NT is a multiple of C) and walks q in steps of NT / C, so the
-  // loop is wave-uniform (idle lanes carry slot -1) and the wave can combine lanes that
-  // share a bin before touching the histogram.
-  static_assert(NT % C == 0, "binning threads keep one column");
-  constexpr int QSTEP = NT / C;
-  constexpr int NIT = (H + 1 + QSTEP - 1) / QSTEP;
-  const int c = tid % C, q0 = tid / C;
+  // Tasks.  Entries (q, ky) and (N - q, ky) share m and mu, and so do (q, -ky) and
+  // (N - q, -ky).  With QUAD (C >= 16) a tile holds two column strips: S_a = ky in
+  // [HW yt, HW yt + HW) and its mirror S_b = [N - HW yt - HW, N - HW yt) (HW = C / 2), so one
+  // task bins the four entries of (q, +-ky).  Strip columns whose mirror lies in another
+  // tile (ky = HW yt and ky = N - HW (yt + 1), the latter -|ky| with |ky| = HW (yt + 1)) are
+  // single-column tasks, which also covers the self-mirrored ky = 0 and ky = N / 2.  Lanes
+  // take LPR = HW + 1 tasks per q row: task s < HW is strip column s (paired with S_b
+  // column HW - s for s >= 1), task HW is S_b column 0.  Without QUAD (C < 16) a task is
+  // one column and the pair (q, N - q).  Every lane walks q in steps of ROWS, so the loop
+  // is wave-uniform (idle lanes carry slot -1) and the wave combines lanes that share a bin
+  // before touching the histogram.
+  constexpr bool QUAD = C >= 16;
+  constexpr int HW = C / 2;
+  constexpr int LPR = QUAD ? HW + 1 : C;
+  constexpr int ROWS = NT / LPR;
+  static_assert(ROWS >= 1, "at least one q row per pass");
+  constexpr int NIT = (H + 1 + ROWS - 1) / ROWS;
+  const int s = tid % LPR;
+  const int q0 = tid < ROWS * LPR ? tid / LPR : H + 1;   // idle lanes never reach a bin
+  // global ky column of tile column c in column tile yt
+  auto gcol = [](int c, int ytile) -> int {
+    if constexpr (QUAD) return c < HW ? HW * ytile + c : N - HW * ytile - 2 * HW + c;
+    else return ytile * C + c;
+  };
   // N = 128: the next tile's columns are loaded into registers while this one is transformed
   // and binned (for larger N the registers would spill or cost a block; at N = 64 a block
   // has a single tile)
   constexpr bool XPF = LH && N == 128;
-  constexpr int XPER = (N * C + NT - 1) / NT;
+  // tile loads: every lane keeps one column (NT is a multiple of C) and rows r0 + k RSTEP,
+  // so its loads share one base address and all of them are in flight before any use
+  static_assert(NT % C == 0, "tile lanes keep one column");
+  constexpr int RSTEP = NT / C;
+  constexpr int XPER = (N + RSTEP - 1) / RSTEP;
+  const int lc = tid % C, lr0 = tid / C;
   float2 xpre[XPF ? XPER : 1];
-  auto load_tile = [&](int t) {
-    const float2* src = spec + (size_t)(t / YT) * N * N + (size_t)(t % YT) * C;
+  auto load_tile = [&](int t, float2* v) {
+    const float2* col = spec + (size_t)(t / YT) * N * N + gcol(lc, t % YT);
 #pragma unroll
     for (int k = 0; k < XPER; ++k) {
-      const int i = tid + k * NT;
-      if (i < N * C) xpre[k] = src[(size_t)(i / C) * N + (i % C)];
+      const int r = lr0 + k * RSTEP;
+      if (N % RSTEP == 0 || r < N) v[k] = col[(size_t)r * N];
+    }
+  };
+  auto store_tile = [&](const float2* v) {
+#pragma unroll
+    for (int k = 0; k < XPER; ++k) {
+      const int r = lr0 + k * RSTEP;
+      if (N % RSTEP == 0 || r < N) buf[r * PS + lc] = v[k];
     }
   };
   if constexpr (XPF) {
-    if (bx < NTILES) load_tile(bx);
+    if (bx < NTILES) load_tile(bx, xpre);
   }
   for (int tile = bx; tile < NTILES; tile += nbx) {
     const int kz = tile / YT, yt = tile % YT;
     const int kz2 = kz * kz;
-    const int y = yt * C + c;
-    const int ky = y <= H ? y : y - N;
-    const int r2 = ky * ky + kz2;
+    // this lane's task: |ky| (tile columns below, after the transform)
+    int kyt;
+    if constexpr (QUAD) {
+      kyt = s < HW ? HW * yt + s : HW * (yt + 1);
+    } else {
+      const int y = yt * C + s;
+      kyt = y <= H ? y : N - y;
+    }
+    const int r2 = kyt * kyt + kz2;
     // bins and window factors do not depend on the transform: the first iteration's table
     // reads are issued before the tile's load and FFT, each later one an iteration ahead
     int slot_next = q0 <= H ? bin_of(q0 * q0 + r2) : -1;
     float wq_next = q0 <= H ? wt(q0) : 0.0f;
-    const float wyz = wt(y) * wt(kz);
+    const float wyz = wt(kyt) * wt(kz);
     if constexpr (XPF) {
-#pragma unroll
-      for (int k = 0; k < XPER; ++k) {
-        const int i = tid + k * NT;
-        if (i < N * C) buf[(i / C) * PS + (i % C)] = xpre[k];
-      }
+      store_tile(xpre);
     } else {
-      stage_tile<N, C, PS, NT>(buf, spec + (size_t)kz * N * N + (size_t)yt * C, N);
+      float2 v[XPER];
+      load_tile(tile, v);
+      store_tile(v);
     }
     __syncthreads();
     if constexpr (XPF) {
-      if (tile + nbx < NTILES) load_tile(tile + nbx);
+      if (tile + nbx < NTILES) load_tile(tile + nbx, xpre);
     }
     lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+    // tile column A of the task, and B with weight fB = 1 for a quad
+    const bool quad = QUAD && s >= 1 && s < HW;
+    const int cA = s;
+    const int cB = quad ? 2 * HW - s : s;
+    const float fB = quad ? 1.0f : 0.0f;
 #pragma unroll 1
     for (int it = 0; it < NIT; ++it) {
-      const int q = q0 + it * QSTEP;
+      const int q = q0 + it * ROWS;
       const int slot = slot_next;
       const float wqi = wq_next;
-      const int qn = q + QSTEP;
+      const int qn = q + ROWS;
       slot_next = qn <= H ? bin_of(qn * qn + r2) : -1;
       wq_next = qn <= H ? wt(qn) : 0.0f;
-      // branch-free: idle lanes (q > H) read rows inside the tile (q < N) and are zeroed
-      // through slot < 0; the k = 0 mode's power is added by the finalize from the FP64
-      // sum of the mesh (mu = 0 there, :37)
+      // branch-free: idle lanes read rows inside the tile and are zeroed through slot < 0;
+      // the k = 0 mode's power is added by the finalize from the FP64 sum of the mesh
+      // (mu = 0 there, :37)
       const int m = q * q + r2;
-      const float2 a = buf[q * PS + c];
-      const float2 bb = buf[(q == 0 ? 0 : N - q) * PS + c];
+      const int qr = q < N ? q : 0;
+      const int qm = (q == 0 || q >= N) ? 0 : N - q;
       const float pair = (q != 0 && q != H) ? 1.0f : 0.0f;
+      const float2 a = buf[qr * PS + cA];
+      const float2 bb = buf[qm * PS + cA];
       float p2 = (a.x * a.x + a.y * a.y) + pair * (bb.x * bb.x + bb.y * bb.y);
+      if constexpr (QUAD) {
+        const float2 c2 = buf[qr * PS + cB];
+        const float2 d2 = buf[qm * PS + cB];
+        p2 += fB * ((c2.x * c2.x + c2.y * c2.y) + pair * (d2.x * d2.x + d2.y * d2.y));
+      }
       p2 = (slot >= 0 && m > 0) ? p2 * (wqi * wyz) : 0.0f;
       // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
       //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
