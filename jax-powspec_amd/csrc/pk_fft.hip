@@ -373,10 +373,20 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       if (N % RSTEP == 0 || r < N) buf[r * PS + lc] = v[k];
     }
   };
+  // Strips narrower than a 128-byte line (QUAD with HW < 16): the blocks of work items t and
+  // t + 8 run on the same XCD at about the same time, so they take the two strips of each
+  // line (tiles 2P and 2P + 1) and the second read finds the line in L2.
+  auto tile_of = [](int t) -> int {
+    if constexpr (QUAD && HW * 8 < 128 && YT % 2 == 0) {
+      if (t < (NTILES / 16) * 16) return 2 * ((t / 16) * 8 + (t % 8)) + ((t % 16) / 8);
+    }
+    return t;
+  };
   if constexpr (XPF) {
-    if (bx < NTILES) load_tile(bx, xpre);
+    if (bx < NTILES) load_tile(tile_of(bx), xpre);
   }
-  for (int tile = bx; tile < NTILES; tile += nbx) {
+  for (int item = bx; item < NTILES; item += nbx) {
+    const int tile = tile_of(item);
     const int kz = tile / YT, yt = tile % YT;
     const int kz2 = kz * kz;
     // this lane's task: |ky| (tile columns below, after the transform)
@@ -402,7 +412,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     }
     __syncthreads();
     if constexpr (XPF) {
-      if (tile + nbx < NTILES) load_tile(tile + nbx, xpre);
+      if (item + nbx < NTILES) load_tile(tile_of(item + nbx), xpre);
     }
     lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
     // tile column A of the task, and B with weight fB = 1 for a quad
