@@ -222,6 +222,23 @@ def test_powspec_vec_parity(gpu, n, edges, window):
     np.testing.assert_allclose(km[fin], oex["kmean"][fin], rtol=1e-6)
 
 
+@pytest.mark.parametrize("n,edges", [(96, "kf_half"), (128, "kf_half"), (192, "kf_half"), (256, "kf_half"),
+                                     (384, "kf_half"), (512, "kf_half"), (128, "narrow_global_hist"),
+                                     (192, "narrow_global_hist")])
+def test_powspec_vec_parity_mid_sizes(gpu, n, edges):
+    # the x pass's strip layouts above n = 64: two 32-column strips (96, 128; register
+    # prefetch at 128), two 8-column strips whose half-line partners are XCD-paired (192,
+    # 256, 384, 512), and the per-block histogram in HBM for > 256 bins
+    rng = np.random.default_rng(n + 5)
+    box = 1200.0
+    d = _delta(rng, n, box)
+    kF = 2 * np.pi / box
+    k_edges = EDGES[edges](n, kF)
+    k3, pk, nm, ex = correlations.powspec_vec(torch.as_tensor(d, device=gpu), box, k_edges, mas="CIC", full=True)
+    ok3, opk, onm, oex = R.powspec_vec(d, box, k_edges, p=2, full=True)
+    pk_check(pk, nm, ex, opk, oex, box, n, sum_abs=float(np.abs(d.astype(np.float64)).sum()))
+
+
 @pytest.mark.parametrize("n", [16, 48, 64])
 def test_powspec_vec_fundamental_parity(gpu, n):
     rng = np.random.default_rng(n)
@@ -255,6 +272,25 @@ def test_estimate_parity(gpu, stencil, n, kind, tile):
     ok3, opk, onm, oex = res[:4]
     # gate before the shot-noise subtraction; the float32 output includes it
     pk_check(pk, nm, ex, opk + [shot, 0, 0], oex, box, n, shot=shot)
+
+
+@pytest.mark.parametrize("stencil,n,npart,kind", [("CIC_REF", 128, 400_000, "kf_half"), ("TSC", 128, 400_000, "fundamental"),
+                                                  ("CIC", 256, 1_000_000, "kf_half"),
+                                                  ("CIC_REF", 256, 1_000_000, "narrow_global_hist")])
+def test_estimate_parity_mid_sizes(gpu, stencil, n, npart, kind):
+    # the fused deposit with several row bundles per x row and up to 1e6 particles
+    rng = np.random.default_rng(n + npart % 97)
+    box = 1500.0
+    p = catalogue(rng, npart, box)
+    kF = 2 * np.pi / box
+    k_edges = None if kind == "fundamental" else EDGES[kind](n, kF)
+    shot = box ** 3 / len(p)
+    t = torch.as_tensor(p, device=gpu)
+    k3, pk, nm, ex = correlations.estimate(t[:, 0], t[:, 1], t[:, 2], box_size=box, n_bins=n, k_edges=k_edges,
+                                           mas=stencil, shot_noise=shot, full=True)
+    window = 3 if stencil == "TSC" else 2
+    res = R.estimate(*p.T, None, box, n, k_edges, stencil=stencil, p=window, shot_noise=shot)
+    pk_check(pk, nm, ex, res[1] + [shot, 0, 0], res[3], box, n, shot=shot)
 
 
 def test_estimate_equals_two_step_path(gpu):
