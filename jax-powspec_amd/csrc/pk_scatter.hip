@@ -60,6 +60,8 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
   const int words = packed ? (g.nkeys + 1) >> 1 : g.nkeys;
   for (int i = tid; i < words; i += CNT_NT) hist[i] = 0u;
   if (tid == 0) wmax_bits = 0u;
+  // block 0 clears the estimate's statistics (written from the scans on; no separate fill)
+  if (bx == 0 && tid < PK_STATS_LEN) j.stats[tid] = 0.0;
   __syncthreads();
   const int64_t lo = (int64_t)bx * g.chunk;
   const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
@@ -223,6 +225,13 @@ __device__ __forceinline__ void scan_top_body(const Job& j) {
   }
 }
 __global__ void __launch_bounds__(SCAN_NT) k_scan_top(Job j) { scan_top_body(j); }
+// all keys in one local block: the local scan, then the top step (its exclusive scan of
+// the single block sum is 0, so no add pass follows)
+__global__ void __launch_bounds__(SCAN_NT) k_scan_single(Job j) {
+  scan_local_body(j, 0);
+  __syncthreads();
+  scan_top_body(j);
+}
 __global__ void __launch_bounds__(SCAN_NT) k_scan_top_jobs(const Job* __restrict__ jobs) {
   PK_JOB_OR_RETURN(jobs, 1);
   scan_top_body(j);
@@ -281,12 +290,6 @@ __global__ void __launch_bounds__(CNT_NT) k_bucket_place(Job j) { bucket_place_b
 __global__ void __launch_bounds__(CNT_NT) k_bucket_place_jobs(const Job* __restrict__ jobs) {
   PK_JOB_OR_RETURN(jobs, j.g.nblk);
   bucket_place_body(j, blockIdx.x);
-}
-
-// stats[0..3] = 0 for every job of a batch (the plain path uses k_fill_f64)
-__global__ void k_zero_stats_jobs(const Job* __restrict__ jobs) {
-  PK_JOB_OR_RETURN(jobs, 1);
-  if (threadIdx.x < PK_STATS_LEN) j.stats[threadIdx.x] = 0.0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -526,15 +529,15 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
   const ScatterArgs& a = j.a;
   const TileGeom& g = j.g;
   const int64_t ncell = (int64_t)a.n * a.n * a.n;
-  {
-    StageTimer t(ST_FILL, s);
-    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(PK_STATS_LEN), 0, s, j.stats, PK_STATS_LEN, 0.0);
-    if (direct) hipLaunchKernelGGL(k_copy_or_zero, dim3(grid_for(ncell, 256, 16384)), dim3(256), 0, s, mesh_in,
-                                   mesh_out, ncell);
-    int e = check_hip(hipGetLastError(), "scatter fill launch");
-    if (e) return e;
-  }
   if (direct) {
+    {
+      StageTimer t(ST_FILL, s);
+      hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(PK_STATS_LEN), 0, s, j.stats, PK_STATS_LEN, 0.0);
+      hipLaunchKernelGGL(k_copy_or_zero, dim3(grid_for(ncell, 256, 16384)), dim3(256), 0, s, mesh_in, mesh_out,
+                         ncell);
+      int e = check_hip(hipGetLastError(), "scatter fill launch");
+      if (e) return e;
+    }
     if (!mesh_out) return set_error(PK_E_INVALID, "direct mode needs a mesh");
     if (a.np > 0) {
       StageTimer t(ST_ATOMIC, s);
@@ -559,9 +562,13 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
     StageTimer t(ST_SCAN, s);
     hipLaunchKernelGGL(k_col_scan, dim3((g.nkeys + COL_NT - 1) / COL_NT), dim3(COL_NT), 0, s, j);
     const int nb2 = (g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    hipLaunchKernelGGL(k_scan_local, dim3(nb2), dim3(SCAN_NT), 0, s, j);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_NT), 0, s, j);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb2), dim3(SCAN_NT), 0, s, j);
+    if (nb2 == 1) {
+      hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_NT), 0, s, j);
+    } else {
+      hipLaunchKernelGGL(k_scan_local, dim3(nb2), dim3(SCAN_NT), 0, s, j);
+      hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_NT), 0, s, j);
+      hipLaunchKernelGGL(k_scan_add, dim3(nb2), dim3(SCAN_NT), 0, s, j);
+    }
     int e = check_hip(hipGetLastError(), "scan launch");
     if (e) return e;
   }
@@ -605,12 +612,6 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
 
 int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
   const int nj = d.njobs;
-  {
-    StageTimer t(ST_FILL, s);
-    hipLaunchKernelGGL(k_zero_stats_jobs, dim3(1, nj), dim3(64), 0, s, jobs);
-    int e = check_hip(hipGetLastError(), "batch stats launch");
-    if (e) return e;
-  }
   {
     StageTimer t(ST_SORT_COUNT, s);
     const size_t lds = (size_t)d.max_nkeys * 4;   // any job may count with 32-bit counters
