@@ -96,37 +96,54 @@ __global__ void __launch_bounds__(CNT_NT) k_bucket_count_jobs(const Job* __restr
 // 2. column scan (offset of each block inside each key) + exclusive scan of key totals
 // ------------------------------------------------------------------------------------
 constexpr int COL_NT = 256;
+constexpr int COL_KEYS = 32;                  // keys per block: one 128-byte segment per row
+constexpr int COL_SUBS = COL_NT / COL_KEYS;   // threads per key, each over a run of rows
+constexpr int COL_ROWS = 32;                  // rows per thread and pass (loads in flight)
 
+// Thread (sub, key) walks rows [sub * COL_ROWS, +COL_ROWS) of each pass of COL_SUBS *
+// COL_ROWS rows: the run sums of a key's threads meet in LDS, so one pass costs one round
+// trip to memory instead of a walk down the whole column.
 __device__ __forceinline__ void col_scan_body(const Job& j, int bx) {
+  __shared__ unsigned part[COL_SUBS][COL_KEYS];
   const int nkeys = j.g.nkeys, nblk = j.g.nblk;
   unsigned* mat = j.mat;
-  const int k = bx * COL_NT + threadIdx.x;
-  if (k >= nkeys) return;
-  // rows in groups of 8: the 8 loads of a group are independent and stay in flight together
-  constexpr int UR = 8;
-  unsigned run = 0;
-  int b = 0;
-  for (; b + UR <= nblk; b += UR) {
-    unsigned c[UR];
+  const int kk = threadIdx.x % COL_KEYS, sub = threadIdx.x / COL_KEYS;
+  const int k = bx * COL_KEYS + kk;
+  const bool on = k < nkeys;
+  unsigned carry = 0;
+  for (int r0 = 0; r0 < nblk; r0 += COL_SUBS * COL_ROWS) {   // block-uniform
+    const int rb = r0 + sub * COL_ROWS;
+    unsigned c[COL_ROWS];
+    unsigned sum = 0;
 #pragma unroll
-    for (int u = 0; u < UR; ++u) c[u] = mat[(size_t)(b + u) * nkeys + k];
-#pragma unroll
-    for (int u = 0; u < UR; ++u) {
-      mat[(size_t)(b + u) * nkeys + k] = run;
-      run += c[u];
+    for (int u = 0; u < COL_ROWS; ++u) {
+      c[u] = (on && rb + u < nblk) ? mat[(size_t)(rb + u) * nkeys + k] : 0u;
+      sum += c[u];
     }
+    part[sub][kk] = sum;
+    __syncthreads();
+    unsigned run = carry, total = 0;
+#pragma unroll
+    for (int t = 0; t < COL_SUBS; ++t) {
+      const unsigned v = part[t][kk];
+      if (t < sub) run += v;
+      total += v;
+    }
+    __syncthreads();   // part is reused by the next pass
+#pragma unroll
+    for (int u = 0; u < COL_ROWS; ++u) {
+      if (on && rb + u < nblk) {
+        mat[(size_t)(rb + u) * nkeys + k] = run;
+        run += c[u];
+      }
+    }
+    carry += total;
   }
-  for (; b < nblk; ++b) {
-    unsigned* p = mat + (size_t)b * nkeys + k;
-    const unsigned c = *p;
-    *p = run;
-    run += c;
-  }
-  j.totals[k] = (int)run;
+  if (on && sub == 0) j.totals[k] = (int)carry;
 }
 __global__ void __launch_bounds__(COL_NT) k_col_scan(Job j) { col_scan_body(j, blockIdx.x); }
 __global__ void __launch_bounds__(COL_NT) k_col_scan_jobs(const Job* __restrict__ jobs) {
-  PK_JOB_OR_RETURN(jobs, (j.g.nkeys + COL_NT - 1) / COL_NT);
+  PK_JOB_OR_RETURN(jobs, (j.g.nkeys + COL_KEYS - 1) / COL_KEYS);
   col_scan_body(j, blockIdx.x);
 }
 
@@ -134,20 +151,27 @@ constexpr int SCAN_NT = 1024;
 constexpr int SCAN_PER = 4;
 constexpr int SCAN_BLOCK = SCAN_NT * SCAN_PER;
 
+// wave scans with shuffles, then the wave totals (two barriers)
 __device__ int block_exclusive_scan(int v, int* sh, int& total) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = 1; off < SCAN_NT; off <<= 1) {
-    const int t = tid >= off ? sh[tid - off] : 0;
-    __syncthreads();
-    sh[tid] += t;
-    __syncthreads();
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
   }
-  total = sh[SCAN_NT - 1];
-  const int incl = sh[tid];
+  if (lane == 63) sh[w] = incl;
   __syncthreads();
-  return incl - v;
+  int before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_NT / 64; ++i) {
+    const int x = sh[i];
+    if (i < w) before += x;
+    tot += x;
+  }
+  total = tot;
+  __syncthreads();
+  return before + incl - v;
 }
 
 // key totals -> key starts: local scans of SCAN_BLOCK keys, a scan of the block sums, add
@@ -209,7 +233,7 @@ __device__ __forceinline__ void scan_top_body(const Job& j) {
     ws += j.wsum[i];
     wm = fmax(wm, j.wmax[i]);
   }
-  const double t = block_sum_f64(ws, shd, SCAN_NT);   // ends with a barrier
+  const double t = block_sum_f64_waves<SCAN_NT>(ws, shd);   // ends with a barrier
   atomicMax(&wmax_bits, (unsigned long long)__double_as_longlong(wm));
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -560,7 +584,7 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
   }
   {
     StageTimer t(ST_SCAN, s);
-    hipLaunchKernelGGL(k_col_scan, dim3((g.nkeys + COL_NT - 1) / COL_NT), dim3(COL_NT), 0, s, j);
+    hipLaunchKernelGGL(k_col_scan, dim3((g.nkeys + COL_KEYS - 1) / COL_KEYS), dim3(COL_NT), 0, s, j);
     const int nb2 = (g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
     if (nb2 == 1) {
       hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_NT), 0, s, j);
@@ -624,7 +648,7 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
   {
     StageTimer t(ST_SCAN, s);
     const int nb2 = (d.max_nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    hipLaunchKernelGGL(k_col_scan_jobs, dim3((d.max_nkeys + COL_NT - 1) / COL_NT, nj), dim3(COL_NT), 0, s, jobs);
+    hipLaunchKernelGGL(k_col_scan_jobs, dim3((d.max_nkeys + COL_KEYS - 1) / COL_KEYS, nj), dim3(COL_NT), 0, s, jobs);
     hipLaunchKernelGGL(k_scan_local_jobs, dim3(nb2, nj), dim3(SCAN_NT), 0, s, jobs);
     hipLaunchKernelGGL(k_scan_top_jobs, dim3(1, nj), dim3(SCAN_NT), 0, s, jobs);
     hipLaunchKernelGGL(k_scan_add_jobs, dim3(nb2, nj), dim3(SCAN_NT), 0, s, jobs);
