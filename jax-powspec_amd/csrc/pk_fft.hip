@@ -519,10 +519,10 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   }
   double acc[NACC];
 #pragma unroll
-  for (int a = 0; a < NACC; ++a) acc[a] = block_sum_f64(part[a], red, FIN_NT);
+  for (int a = 0; a < NACC; ++a) acc[a] = block_sum_f64_waves<FIN_NT>(part[a], red);
   double dcp = 0.0;
   for (int g = threadIdx.x; dc_part != nullptr && g < Gz; g += FIN_NT) dcp += dc_part[g];
-  const double dc = dc_part != nullptr ? block_sum_f64(dcp, red, FIN_NT) : 0.0;
+  const double dc = dc_part != nullptr ? block_sum_f64_waves<FIN_NT>(dcp, red) : 0.0;
   if (threadIdx.x != 0) return;
   // spectrum of rho - c (fused assignment): delta_k = rho_k * n^3 / M for k != 0
   const double sc = scale_stats ? scale_stats[1] * scale_stats[1] : 1.0;

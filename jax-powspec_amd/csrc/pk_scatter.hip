@@ -398,19 +398,32 @@ __global__ void __launch_bounds__(ATM_NT) k_atomic_deposit(ScatterArgs a, const 
 // ------------------------------------------------------------------------------------
 // mass finalize: stats[0] = M (fixed-order sum of tile masses + atomic part), stats[1] = n^3 / M
 // ------------------------------------------------------------------------------------
+// fixed-order FP64 sum of the tile masses: each thread adds its slots in order (eight
+// loads in flight per step), then wave butterflies and the wave totals
+constexpr int MASS_NT = 1024;
 __device__ __forceinline__ void mass_finalize_body(const Job& j, int ntiles) {
-  __shared__ double red[ATM_NT];
+  __shared__ double red[MASS_NT / 64 + 1];
+  constexpr int MU = 8;
   double s = 0.0;
-  for (int i = threadIdx.x; i < ntiles; i += ATM_NT) s += j.tile_mass[i];
-  const double m = block_sum_f64(s, red, ATM_NT) + j.stats[2];
+  for (int i0 = threadIdx.x; i0 < ntiles; i0 += MASS_NT * MU) {
+    double v[MU];
+#pragma unroll
+    for (int u = 0; u < MU; ++u) {
+      const int i = i0 + u * MASS_NT;
+      v[u] = i < ntiles ? j.tile_mass[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < MU; ++u) s += v[u];
+  }
+  const double m = block_sum_f64_waves<MASS_NT>(s, red) + j.stats[2];
   if (threadIdx.x == 0) {
     const double n = (double)j.a.n;
     j.stats[0] = m;
     j.stats[1] = n * n * n / m;
   }
 }
-__global__ void __launch_bounds__(ATM_NT) k_mass_finalize(Job j, int ntiles) { mass_finalize_body(j, ntiles); }
-__global__ void __launch_bounds__(ATM_NT) k_mass_finalize_jobs(const Job* __restrict__ jobs) {
+__global__ void __launch_bounds__(MASS_NT) k_mass_finalize(Job j, int ntiles) { mass_finalize_body(j, ntiles); }
+__global__ void __launch_bounds__(MASS_NT) k_mass_finalize_jobs(const Job* __restrict__ jobs) {
   PK_JOB_OR_RETURN(jobs, 1);
   mass_finalize_body(j, j.g.ntiles);
 }
@@ -542,7 +555,7 @@ size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np) {
 
 int mass_finalize(const Job& j, hipStream_t s) {
   StageTimer t(ST_MASS, s);
-  hipLaunchKernelGGL(k_mass_finalize, dim3(1), dim3(ATM_NT), 0, s, j, j.tile_mass ? j.g.ntiles : 0);
+  hipLaunchKernelGGL(k_mass_finalize, dim3(1), dim3(MASS_NT), 0, s, j, j.tile_mass ? j.g.ntiles : 0);
   return check_hip(hipGetLastError(), "mass finalize launch");
 }
 
@@ -665,7 +678,7 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
 
 int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s) {
   StageTimer t(ST_MASS, s);
-  hipLaunchKernelGGL(k_mass_finalize_jobs, dim3(1, njobs), dim3(ATM_NT), 0, s, jobs);
+  hipLaunchKernelGGL(k_mass_finalize_jobs, dim3(1, njobs), dim3(MASS_NT), 0, s, jobs);
   return check_hip(hipGetLastError(), "batch mass finalize launch");
 }
 
