@@ -217,6 +217,59 @@ int pk_batch_run(pk_batch* batch, void* stream);
 int pk_batch_tile(pk_batch* batch, int* tile);
 int pk_batch_destroy(pk_batch* batch);
 
+/* --- seeded lognormal mocks -----------------------------------------------------------
+ * The reference's mock pipeline (tests/create_lognormal.py:42-55): a Gaussian random
+ * field from a tabulated P(k) (src/gauss_field.py:5-80), its inverse FFT
+ * (scipy.fft.irfftn, :47), the lognormal transform exp(b1 g) normalised to mean 1
+ * (:50-51) and Poisson population with triangular in-cell offsets
+ * (src/gauss_field.py:83-110, src/populate_field.py:4-29).
+ * Random numbers come from counter-based Philox4x32-10 streams keyed by the 64-bit seed
+ * (the reference uses NumPy's MT19937 and JAX's threefry, so a given seed does not give
+ * the reference's values): mode m = (kxx n + kyy)(n/2+1) + kzz draws from block
+ * (m, 0, 1), cell c (C order) from blocks (c, j, 2), particle p from block (p, 0, 3),
+ * each block with counter words (index mod 2^32, index >> 32, j, tag) and key
+ * (seed mod 2^32, seed >> 32).  Results do not depend on launch shapes or devices.
+ *   - kf, pkf: FP64 device arrays of nk >= 2 entries, kf strictly increasing; P(|k|) is
+ *     interpolated linearly between the bracketing entries (end segments extrapolate, as
+ *     the reference) and must be >= 0 wherever a mode falls (sqrt of a negative is NaN).
+ *   - the spectrum is the plan's layout [n/2+1][n][n] ([kz][kx][ky]): entry (kz, i, j)
+ *     equals the reference's delta_k[i, j, kz]; workspace: pk_gaussian_workspace_size.
+ */
+#define PK_FIELD_DELTA 0   /* the Gaussian field itself */
+#define PK_FIELD_PSI_Z 1   /* its Zel'dovich displacement along z, i k_z delta / k^2 (length
+                              units, k_z = n/2 plane zeroed); no reference counterpart that runs:
+                              src/populate_field.py:57-69 is the broken field-level version */
+int pk_gaussian_workspace_size(int n, size_t* bytes);
+/* gaussian_field(grid, kf, Pkf, Rayleigh_sampling, seed, BoxSize), src/gauss_field.py:5-80 */
+int pk_gaussian_field(pk_plan* plan, double box_size, const double* kf, const double* pkf, int nk, int rayleigh,
+                      uint64_t seed, void* spectrum, void* workspace, size_t workspace_bytes, void* stream);
+/* mesh = numpy.fft.irfftn(spectrum) (tests/create_lognormal.py:47): 1/n^3 normalisation,
+ * imaginary parts of the kz = 0 and kz = n/2 entries ignored after the x and y inverse
+ * transforms.  scratch is a spectrum-sized buffer and may equal spectrum (then the
+ * spectrum is overwritten); otherwise the spectrum is only read. */
+int pk_c2r(pk_plan* plan, const void* spectrum, void* scratch, float* mesh, void* stream);
+/* irfftn of the generated field (component PK_FIELD_DELTA) or of its z displacement
+ * (PK_FIELD_PSI_Z) without writing the spectrum: the modes are produced inside the first
+ * inverse pass.  Equals pk_gaussian_field followed by pk_c2r (and the psi_z factor). */
+int pk_gaussian_mesh(pk_plan* plan, double box_size, const double* kf, const double* pkf, int nk, int rayleigh,
+                     uint64_t seed, int component, void* scratch, float* mesh, void* workspace,
+                     size_t workspace_bytes, void* stream);
+/* populate_field(rho, n_bins, box_size, density, seed), src/gauss_field.py:90-110: in two
+ * calls on the same stream, workspace and arguments.  With bias != 0 the field is a
+ * Gaussian field g and rho = exp(bias g) (tests/create_lognormal.py:50); with bias = 0 it is
+ * rho itself.  Rates lambda = rho (L/n)^3 density / mean(rho) (FP64); Poisson counts per
+ * cell (rates <= 0 or NaN give 0).  pk_populate_count writes the particle total to *total
+ * (device int64).  pk_populate_place writes min(total, capacity) particles as float32
+ * pos[p][3] in C order of their cells: cell centre + triangular offset, plus
+ * growth_rate * psi_z[cell] along z when psi_z is given (redshift-space shift), wrapped
+ * into [0, L).  The reference orders cells by decreasing rho instead (a permutation). */
+int pk_populate_workspace_size(int n, size_t* bytes);
+int pk_populate_count(const float* field, int n, double box_size, double density, double bias, uint64_t seed,
+                      void* workspace, size_t workspace_bytes, int64_t* total, void* stream);
+int pk_populate_place(const float* field, const float* psi_z, double growth_rate, int n, double box_size,
+                      double bias, uint64_t seed, void* workspace, size_t workspace_bytes, float* pos,
+                      int64_t capacity, void* stream);
+
 /* --- per-stage timing (bench / profiling; no reference counterpart) ----------------
  * When enabled, each kernel stage is bracketed by HIP events on its own stream.
  * pk_timing_read synchronises those events, returns total ms and launch counts per stage

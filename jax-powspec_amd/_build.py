@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libpowspec_amd.so"
 ARCH = "gfx950"
-SOURCES = ["pk_capi.hip", "pk_scatter.hip", "pk_fft.hip", "pk_fused.hip", "pk_adjoint.hip"]
+SOURCES = ["pk_capi.hip", "pk_scatter.hip", "pk_fft.hip", "pk_fused.hip", "pk_adjoint.hip", "pk_mock.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

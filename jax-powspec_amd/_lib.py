@@ -33,6 +33,8 @@ _f = C.c_float
 _d = C.c_double
 _sz = C.c_size_t
 _psz = C.POINTER(C.c_size_t)
+FIELD_DELTA = 0
+FIELD_PSI_Z = 1
 BATCH_GRAPH = 1
 BATCH_CAPACITY_GRIDS = 2
 
@@ -74,6 +76,13 @@ SIGNATURES = [
     ("pk_batch_run", _i, [_vp, _vp]),
     ("pk_batch_tile", _i, [_vp, _vp]),
     ("pk_batch_destroy", _i, [_vp]),
+    ("pk_gaussian_workspace_size", _i, [_i, _psz]),
+    ("pk_gaussian_field", _i, [_vp, _d, _vp, _vp, _i, _i, C.c_uint64, _vp, _vp, _sz, _vp]),
+    ("pk_c2r", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("pk_gaussian_mesh", _i, [_vp, _d, _vp, _vp, _i, _i, C.c_uint64, _i, _vp, _vp, _vp, _sz, _vp]),
+    ("pk_populate_workspace_size", _i, [_i, _psz]),
+    ("pk_populate_count", _i, [_vp, _i, _d, _d, _d, C.c_uint64, _vp, _sz, _vp, _vp]),
+    ("pk_populate_place", _i, [_vp, _vp, _d, _i, _d, _d, C.c_uint64, _vp, _sz, _vp, _i64, _vp]),
     ("pk_timing_enable", _i, [_i]),
     ("pk_timing_read", _i, [_vp, _vp, _i]),
     ("pk_stage_name", C.c_char_p, [_i]),

@@ -872,3 +872,115 @@ const char* pk_stage_name(int stage) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// seeded mock generator
+// ------------------------------------------------------------------------------------
+namespace {
+int make_field_gen(pk_plan* p, double box, const double* kf, const double* pkf, int nk, int rayleigh, uint64_t seed,
+                   void* ws, size_t ws_bytes, FieldGen* g, hipStream_t s) {
+  int e = check_plan(p);
+  if (e) return e;
+  if (!(box > 0.0)) return set_error(PK_E_INVALID, "box_size must be > 0");
+  if (!kf || !pkf) return set_error(PK_E_INVALID, "null kf / Pkf table");
+  if (nk < 2) return set_error(PK_E_INVALID, "the P(k) table needs at least 2 entries (nk=%d)", nk);
+  if (rayleigh != 0 && rayleigh != 1) return set_error(PK_E_INVALID, "Rayleigh_sampling must be 0 or 1");
+  const size_t need = align_up(gauss_table_len(p->n) * sizeof(float));
+  if (!ws || ws_bytes < need) return set_error(PK_E_WORKSPACE, "Gaussian-field workspace %zu < %zu bytes", ws_bytes, need);
+  float* amp = static_cast<float*>(ws);
+  e = gauss_table(p->n, box, kf, pkf, nk, amp, s);
+  if (e) return e;
+  g->amp = amp;
+  g->k0 = (uint32_t)seed;
+  g->k1 = (uint32_t)(seed >> 32);
+  g->n = p->n;
+  g->rayleigh = rayleigh;
+  g->l2pi = (float)(box / (2.0 * M_PI));
+  return PK_OK;
+}
+
+int check_pop(const float* field, int n, double box) {
+  if (!field) return set_error(PK_E_INVALID, "null field");
+  if (n < 1 || n > 4096) return set_error(PK_E_INVALID, "mesh side n=%d out of range", n);
+  if (!(box > 0.0)) return set_error(PK_E_INVALID, "box_size must be > 0");
+  return PK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pk_gaussian_workspace_size(int n, size_t* bytes) {
+  if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
+  int e = check_n(n);
+  if (e) return e;
+  *bytes = align_up(gauss_table_len(n) * sizeof(float));
+  return PK_OK;
+}
+
+int pk_gaussian_field(pk_plan* p, double box_size, const double* kf, const double* pkf, int nk, int rayleigh,
+                      uint64_t seed, void* spectrum, void* workspace, size_t workspace_bytes, void* stream) {
+  hipStream_t s = as_stream(stream);
+  FieldGen g{};
+  int e = make_field_gen(p, box_size, kf, pkf, nk, rayleigh, seed, workspace, workspace_bytes, &g, s);
+  if (e) return e;
+  if (!spectrum) return set_error(PK_E_INVALID, "null spectrum");
+  return gauss_spectrum(g, static_cast<float2*>(spectrum), s);
+}
+
+int pk_c2r(pk_plan* p, const void* spectrum, void* scratch, float* mesh, void* stream) {
+  int e = check_plan(p);
+  if (e) return e;
+  if (!spectrum || !scratch || !mesh) return set_error(PK_E_INVALID, "null spectrum / scratch / mesh");
+  FieldGen g{};
+  g.n = p->n;
+  return fft_c2r(p->n, C2R_SPECTRUM, static_cast<const float2*>(spectrum), static_cast<float2*>(scratch), g, p->tw,
+                 mesh, as_stream(stream));
+}
+
+int pk_gaussian_mesh(pk_plan* p, double box_size, const double* kf, const double* pkf, int nk, int rayleigh,
+                     uint64_t seed, int component, void* scratch, float* mesh, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  if (component != PK_FIELD_DELTA && component != PK_FIELD_PSI_Z)
+    return set_error(PK_E_INVALID, "unknown field component %d", component);
+  if (!scratch || !mesh) return set_error(PK_E_INVALID, "null scratch / mesh");
+  hipStream_t s = as_stream(stream);
+  FieldGen g{};
+  int e = make_field_gen(p, box_size, kf, pkf, nk, rayleigh, seed, workspace, workspace_bytes, &g, s);
+  if (e) return e;
+  return fft_c2r(p->n, component == PK_FIELD_DELTA ? C2R_DELTA : C2R_PSI_Z, nullptr, static_cast<float2*>(scratch), g,
+                 p->tw, mesh, s);
+}
+
+int pk_populate_workspace_size(int n, size_t* bytes) {
+  if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
+  if (n < 1 || n > 4096) return set_error(PK_E_INVALID, "mesh side n=%d out of range", n);
+  *bytes = populate_ws_bytes(n);
+  return PK_OK;
+}
+
+int pk_populate_count(const float* field, int n, double box_size, double density, double bias, uint64_t seed,
+                      void* workspace, size_t workspace_bytes, int64_t* total, void* stream) {
+  int e = check_pop(field, n, box_size);
+  if (e) return e;
+  if (!(density >= 0.0)) return set_error(PK_E_INVALID, "density must be >= 0");
+  const size_t need = populate_ws_bytes(n);
+  if (!workspace || workspace_bytes < need)
+    return set_error(PK_E_WORKSPACE, "population workspace %zu < %zu bytes", workspace_bytes, need);
+  return populate_count(field, n, box_size, density, bias, seed, workspace, total, as_stream(stream));
+}
+
+int pk_populate_place(const float* field, const float* psi_z, double growth_rate, int n, double box_size, double bias,
+                      uint64_t seed, void* workspace, size_t workspace_bytes, float* pos, int64_t capacity,
+                      void* stream) {
+  int e = check_pop(field, n, box_size);
+  if (e) return e;
+  if (capacity < 0) return set_error(PK_E_INVALID, "negative capacity");
+  if (capacity > 0 && !pos) return set_error(PK_E_INVALID, "null positions");
+  const size_t need = populate_ws_bytes(n);
+  if (!workspace || workspace_bytes < need)
+    return set_error(PK_E_WORKSPACE, "population workspace %zu < %zu bytes", workspace_bytes, need);
+  return populate_place(field, psi_z, growth_rate, n, box_size, bias, seed, workspace, pos, capacity,
+                        as_stream(stream));
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@
 //   l = 0, 2, 4 weights (:45-47); every stored half-spectrum entry counts once (:48).
 #include "pk_common.h"
 #include "pk_internal.h"
+#include "pk_mock.h"
 
 namespace pk {
 
@@ -702,6 +703,65 @@ k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, fl
 }
 
 // ------------------------------------------------------------------------------------
+// inverse x pass of numpy.fft.irfftn: columns along kx at fixed (kz, ky), inverse DFT
+// via the forward FFT of the conjugate.  The input is a stored half spectrum
+// (SRC = C2R_SPECTRUM, spec_in) or modes generated in the tile (C2R_DELTA: the Gaussian
+// field, C2R_PSI_Z: its Zel'dovich z displacement), so a generated spectrum is never
+// written or read back.  irfftn's 1/n^3 and the doubling of the interior kz planes (the
+// half-spectrum -> real rows step below sums Re B_k once per interior k) are folded in
+// here; the y pass (inverse) and the z pass (C2R) follow.
+// ------------------------------------------------------------------------------------
+template <int N, int SRC>
+__global__ void __launch_bounds__(FFTCfg<N>::NTX)
+k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, const float2* __restrict__ twg,
+                FieldGen g) {
+  using Cf = FFTCfg<N>;
+  constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = (N / 2 + 1) * YT;
+  constexpr int H = N / 2;
+  __shared__ float2 buf[N * PS];
+  __shared__ float2 tw[N];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  const float s_edge = (float)(1.0 / ((double)N * (double)N * (double)N));
+  const float s_int = 2.0f * s_edge;
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int kz = tile / YT, yt = tile % YT;
+    const float sc = (kz == 0 || kz == H) ? s_edge : s_int;
+    if constexpr (SRC == C2R_SPECTRUM) {
+      const float2* src = spec_in + (size_t)kz * N * N + (size_t)yt * C;
+      constexpr int TOT = N * C, PER = (TOT + NT - 1) / NT;
+      float2 v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) v[k] = src[(size_t)(i / C) * N + (i % C)];
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) buf[(i / C) * PS + (i % C)] = make_float2(sc * v[k].x, -sc * v[k].y);
+      }
+    } else {
+      for (int i = tid; i < N * C; i += NT) {
+        const int r = i / C, c = i % C;
+        const float2 v = SRC == C2R_DELTA ? gauss_mode(g, r, yt * C + c, kz) : psi_z_mode(g, r, yt * C + c, kz);
+        buf[r * PS + c] = make_float2(sc * v.x, -sc * v.y);
+      }
+    }
+    __syncthreads();
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+    float2* ob = out + (size_t)kz * N * N + (size_t)yt * C;
+    for (int i = tid; i < N * C; i += NT) {
+      const float2 v = buf[(i / C) * PS + (i % C)];
+      ob[(size_t)(i / C) * N + (i % C)] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------
 template <typename K>
@@ -805,6 +865,26 @@ template <int N> struct FFTLaunch {
     hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, bspec, tw);
     hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)bspec, tw, gmesh);
     return check_hip(hipGetLastError(), "adjoint FFT launch");
+  }
+  template <int SRC>
+  static int xinv(const float2* in, float2* scratch, const float2* tw, const FieldGen& g, hipStream_t s) {
+    static int gx = 0;
+    if (!gx) gx = persistent_grid(k_xpass_inverse<N, SRC>, Cf::NTX, XTILES);
+    hipLaunchKernelGGL((k_xpass_inverse<N, SRC>), dim3(gx), dim3(Cf::NTX), 0, s, in, scratch, tw, g);
+    return check_hip(hipGetLastError(), "inverse x pass launch");
+  }
+  static int c2r(int src, const float2* in, float2* scratch, const FieldGen& g, const float2* tw, float* mesh,
+                 hipStream_t s) {
+    static int gy = 0, gz = 0;
+    if (!gy) gy = persistent_grid(k_ypass_inverse<N>, Cf::NTY, YTILES);
+    if (!gz) gz = persistent_grid(k_zpass_c2r<N>, Cf::NTZ, ZTILES);
+    int e = src == C2R_SPECTRUM ? xinv<C2R_SPECTRUM>(in, scratch, tw, g, s)
+            : src == C2R_DELTA  ? xinv<C2R_DELTA>(in, scratch, tw, g, s)
+                                : xinv<C2R_PSI_Z>(in, scratch, tw, g, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw);
+    hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw, mesh);
+    return check_hip(hipGetLastError(), "inverse FFT launch");
   }
   static int xpass_bin_jobs(const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s) {
     StageTimer t(ST_XPASS_BIN, s);
@@ -931,6 +1011,16 @@ int fft_xpass_bin_jobs(int n, const Job* jobs, int njobs, int max_nb, int gx, co
 int fft_adjoint(int n, const Job& j, const float2* tw, const float* coef, float2* bspec, float* gmesh, hipStream_t s) {
   switch (n) {
 #define PK_CASE(V) case V: return FFTLaunch<V>::adjoint(j, tw, coef, bspec, gmesh, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int fft_c2r(int n, int src, const float2* spec_in, float2* scratch, const FieldGen& g, const float2* tw, float* mesh,
+            hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::c2r(src, spec_in, scratch, g, tw, mesh, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }

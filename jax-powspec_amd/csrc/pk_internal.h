@@ -148,6 +148,38 @@ bool fused_supported(int n, const TileGeom& g);
 int fused_deposit_zpass(int n, const Job& j, const float2* tw, hipStream_t s);
 int fused_deposit_zpass_jobs(int n, int ty, const Job* jobs, int njobs, int ntiles, const float2* tw, hipStream_t s);
 
+// ---- seeded mock generator (pk_mock.hip; the inverse x pass in pk_fft.hip) ---------------
+// Gaussian half-spectrum modes: amp[m] = sqrt(P (n^2/L)^3) at |k| = k_F sqrt(m),
+// m = 0 .. 3 (n/2)^2; Philox key (k0, k1) from the seed
+struct FieldGen {
+  const float* amp;
+  uint32_t k0, k1;
+  int n;
+  int rayleigh;
+  float l2pi;   // L / (2 pi): physical units of the displacement component
+};
+// sources of the inverse transform
+enum { C2R_SPECTRUM = 0, C2R_DELTA = 1, C2R_PSI_Z = 2 };
+// numpy.fft.irfftn of a half spectrum ([kz][kx][ky]; src == C2R_SPECTRUM, read from
+// spec_in, which may equal scratch) or of the generated delta / psi_z modes; scratch is a
+// half-spectrum buffer, mesh [n][n][n] float32
+int fft_c2r(int n, int src, const float2* spec_in, float2* scratch, const FieldGen& g, const float2* tw, float* mesh,
+            hipStream_t s);
+
+// amplitude table length 3 (n/2)^2 + 1 and its fill from (kf, pkf) (FP64 device arrays)
+size_t gauss_table_len(int n);
+int gauss_table(int n, double box, const double* kf, const double* pkf, int nk, float* amp, hipStream_t s);
+// the half spectrum [kz][kx][ky] of the generated field
+int gauss_spectrum(const FieldGen& g, float2* spec, hipStream_t s);
+// Poisson population of a field (rho = exp(bias * field), or the field when bias = 0):
+// count writes per-block offsets and the rate scale into ws and the total into *total;
+// place (after count, same ws and arguments) writes particles [total][3]
+size_t populate_ws_bytes(int n);
+int populate_count(const float* field, int n, double box, double density, double bias, uint64_t seed, void* ws,
+                   int64_t* total, hipStream_t s);
+int populate_place(const float* field, const float* psi, double growth, int n, double box, double bias, uint64_t seed,
+                   void* ws, float* pos, int64_t capacity, hipStream_t s);
+
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
 }  // namespace pk
