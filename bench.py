@@ -434,6 +434,69 @@ def bench_served(dev, plain_sample=300):
                     "one request at a time (capacity measured on the first requests)"}
 
 
+MOCK_KF = np.linspace(1e-4, 2.0, 4056)
+MOCK_PKF = 2e4 * (MOCK_KF / 0.05) ** -1.5 * np.exp(-(MOCK_KF / 0.5) ** 2)
+
+
+def mock_bytes(n, npart, rsd):
+    """Minimum HBM bytes of one mock: field pass (write half spectrum), inverse y (read +
+    write), C2R (read half spectrum, write mesh), three population reads of the field,
+    one-byte counts written and read, cell ids written and read, records written; with
+    RSD a second field pass and the psi_z reads."""
+    half = 8 * n * n * (n // 2 + 1)
+    mesh = 4 * n ** 3
+    field = half + 2 * half + half + mesh
+    pop = 3 * mesh + 2 * n ** 3 + 8 * npart + 12 * npart
+    return (2 if rsd else 1) * field + pop + (4 * npart if rsd else 0)
+
+
+def bench_mock(dev, reps=5, cpu=True):
+    """Seeded lognormal mock at the headline size (tests/create_lognormal.py:42-55 on the
+    GPU): Gaussian field of P(k) -> irfftn -> exp(b g) -> Poisson population, 512^3 cells,
+    L = 1500 Mpc/h, 2e7 particles on average; plus the Zel'dovich redshift-space shift."""
+    import torch
+    from jax_powspec_amd import gauss_field as GF
+
+    n, box, bias, f = N_MESH, BOX, 1.1, 0.7
+    out = {"workload": f"lognormal mock {n}^3 cells, L={BOX:g} Mpc/h, nbar={NBAR:.3e}, b={bias}, "
+                       f"f={f} (RSD run)", "mesh": n}
+    for growth in (0.0, f):
+        GF.lognormal_mock(n, MOCK_KF, MOCK_PKF, box, NBAR, bias, 11, growth_rate=growth)
+        torch.cuda.synchronize()
+        ts = []
+        for r in range(reps):
+            t0 = time.perf_counter()
+            pos = GF.lognormal_mock(n, MOCK_KF, MOCK_PKF, box, NBAR, bias, 100 + r, growth_rate=growth)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ms = float(np.median(ts)) * 1e3
+        npart = int(pos.shape[0])
+        key = "rsd" if growth else "real"
+        gbs = mock_bytes(n, npart, bool(growth)) / (ms * 1e-3) / 1e9
+        out[key] = {"ms": round(ms, 3), "particles": npart, "particles_per_s": round(npart / (ms * 1e-3), 1),
+                    "algorithmic_gbs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+        del pos
+        torch.cuda.empty_cache()
+    if cpu:
+        # the oracle's NumPy pipeline on a 128^3 sample of the same cell size, P(k), density
+        # and bias (same rates per cell), per-cell time scaled to 512^3
+        from oracle import mocks as M
+        ns = 128
+        bs = box * ns / n
+        t0 = time.perf_counter()
+        d = M.gaussian_field(ns, MOCK_KF, MOCK_PKF, 0, 11, bs)
+        g = M.irfftn(d).astype(np.float32)
+        p, _ = M.populate(g, ns, bs, NBAR, 12, bias=bias)
+        dt = time.perf_counter() - t0
+        scaled = dt * (n / ns) ** 3
+        out["cpu_baseline"] = {"ms_scaled": round(scaled * 1e3, 1), "unit": "ms per mock", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle NumPy pipeline at {ns}^3 (L={bs:g}, {p.shape[0]} particles, "
+                                         f"{dt:.2f} s), per-cell time scaled to {n}^3"}
+        out["gpu_over_cpu"] = round(scaled * 1e3 / out["real"]["ms"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -449,6 +512,8 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving benchmark")
     ap.add_argument("--no-small", action="store_true", help="skip the 128^3 / 1e6-particle plain case")
     ap.add_argument("--only-served", action="store_true", help="run only the serving benchmark")
+    ap.add_argument("--no-mock", action="store_true", help="skip the mock-generator measurement")
+    ap.add_argument("--only-mock", action="store_true", help="run only the mock-generator measurement")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the calibrated HBM traffic summary of two rocprofv3 --pmc runs and exit")
     ap.add_argument("--pmc-command", default="", help="command line recorded in the --pmc-summary header")
@@ -486,6 +551,9 @@ def main():
         return
     if args.only_served:
         print(json.dumps({"served": bench_served(dev)}), flush=True)
+        return
+    if args.only_mock:
+        print(json.dumps({"mock": bench_mock(dev, cpu=not args.no_cpu)}), flush=True)
         return
     log("generating catalogue ...")
     pos = make_catalogue(N_MESH, BOX, NBAR, dev)
@@ -562,6 +630,11 @@ def main():
     if not args.no_served:
         served = bench_served(dev)
         torch.cuda.empty_cache()
+    mock = None
+    if not args.no_mock:
+        mock = bench_mock(dev, cpu=not args.no_cpu)
+        log(f"mock: {mock}")
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -586,6 +659,7 @@ def main():
         "small": small,
         "batched": batched,
         "served": served,
+        "mock": mock,
     }
     print(json.dumps(out), flush=True)
 

@@ -901,7 +901,7 @@ int make_field_gen(pk_plan* p, double box, const double* kf, const double* pkf, 
 
 int check_pop(const float* field, int n, double box) {
   if (!field) return set_error(PK_E_INVALID, "null field");
-  if (n < 1 || n > 4096) return set_error(PK_E_INVALID, "mesh side n=%d out of range", n);
+  if (n < 1 || n > 1024) return set_error(PK_E_INVALID, "mesh side n=%d out of range (1..1024)", n);
   if (!(box > 0.0)) return set_error(PK_E_INVALID, "box_size must be > 0");
   return PK_OK;
 }
@@ -953,7 +953,7 @@ int pk_gaussian_mesh(pk_plan* p, double box_size, const double* kf, const double
 
 int pk_populate_workspace_size(int n, size_t* bytes) {
   if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
-  if (n < 1 || n > 4096) return set_error(PK_E_INVALID, "mesh side n=%d out of range", n);
+  if (n < 1 || n > 1024) return set_error(PK_E_INVALID, "mesh side n=%d out of range (1..1024)", n);
   *bytes = populate_ws_bytes(n);
   return PK_OK;
 }

@@ -5,7 +5,9 @@
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), so any thread can produce any mode,
 // cell or particle in any order, and a result does not depend on the launch shape.
 //   field mode m = (kxx n + kyy)(n/2 + 1) + kzz : block (m, 0, TAG_FIELD)
-//   Poisson cell c (C order)                     : blocks (c, j, TAG_POISSON), j = 0, 1, ...
+//   Poisson cell c (C order), inversion uniform  : block (c >> 1, 0, TAG_POISSON), words
+//                                                  (x, y) for even c, (z, w) for odd c
+//   Poisson cell c, rejection rounds (lam >= 10) : blocks (c, j, TAG_PTRS), j = 0, 1, ...
 //   particle p (output order)                    : block (p, 0, TAG_DISP)
 // with key (seed mod 2^32, seed >> 32).
 #pragma once
@@ -19,6 +21,7 @@ namespace pk {
 constexpr uint32_t TAG_FIELD = 1;
 constexpr uint32_t TAG_POISSON = 2;
 constexpr uint32_t TAG_DISP = 3;
+constexpr uint32_t TAG_PTRS = 4;
 constexpr int POISSON_INV_KMAX = 100;   // inversion bound (lambda < 10: tail beyond < 1e-60)
 constexpr int POISSON_PTRS_MAX_IT = 64; // rejection rounds before the fallback floor(lambda)
 constexpr double POISSON_LAM_MAX = 1073741824.0;  // rates are clamped to 2^30 (counts fit int32)
@@ -121,24 +124,8 @@ __device__ __forceinline__ double log_factorial(double k) {
   return (x - 0.5) * log(x) - x + 0.91893853320467274178 + corr;
 }
 
-// Poisson draw of rate lam for cell c: inversion (sequential search of the CDF in FP64)
-// for lam < 10, Hormann's PTRS transformed rejection (1993) above.  lam <= 0 or NaN -> 0;
-// lam above 2^30 (or infinite) is taken as 2^30.
-__device__ __forceinline__ int poisson_draw(double lam, uint64_t c, uint32_t k0, uint32_t k1) {
-  if (!(lam > 0.0)) return 0;
-  lam = fmin(lam, POISSON_LAM_MAX);
-  if (lam < 10.0) {
-    const Philox4 w = rng_block(c, 0, TAG_POISSON, k0, k1);
-    const double u = uniform_f64(w.x, w.y);
-    double p = exp(-lam), F = p;
-    int k = 0;
-    while (u > F && k < POISSON_INV_KMAX) {
-      ++k;
-      p *= lam / (double)k;
-      F += p;
-    }
-    return k;
-  }
+// Hormann's PTRS transformed rejection (1993) for 10 <= lam <= 2^30
+__device__ __forceinline__ int poisson_ptrs(double lam, uint64_t c, uint32_t k0, uint32_t k1) {
   const double slam = sqrt(lam), loglam = log(lam);
   const double b = 0.931 + 2.53 * slam;
   const double a = -0.059 + 0.02483 * b;
@@ -146,7 +133,7 @@ __device__ __forceinline__ int poisson_draw(double lam, uint64_t c, uint32_t k0,
   const double vr = 0.9277 - 3.6224 / (b - 2.0);
   const double lia = log(invalpha);
   for (int j = 0; j < POISSON_PTRS_MAX_IT; ++j) {
-    const Philox4 w = rng_block(c, (uint32_t)j, TAG_POISSON, k0, k1);
+    const Philox4 w = rng_block(c, (uint32_t)j, TAG_PTRS, k0, k1);
     const double U = uniform_f64(w.x, w.y) - 0.5;
     const double V = uniform_f64(w.z, w.w);
     const double us = 0.5 - fabs(U);
@@ -156,6 +143,31 @@ __device__ __forceinline__ int poisson_draw(double lam, uint64_t c, uint32_t k0,
     if (log(V) + lia - log(a / (us * us) + b) <= -lam + kk * loglam - log_factorial(kk)) return (int)kk;
   }
   return (int)floor(lam);
+}
+
+// Poisson draw of rate lam for cell c with inversion uniform u: inversion (sequential
+// search of the CDF in FP64) for lam < 10, PTRS above.  lam <= 0 or NaN -> 0; lam above
+// 2^30 (or infinite) is taken as 2^30.
+__device__ __forceinline__ int poisson_draw(double lam, double u, uint64_t c, uint32_t k0, uint32_t k1) {
+  if (!(lam > 0.0)) return 0;
+  lam = fmin(lam, POISSON_LAM_MAX);
+  if (lam >= 10.0) return poisson_ptrs(lam, c, k0, k1);
+  double p = exp(-lam), F = p;
+  int k = 0;
+  while (u > F && k < POISSON_INV_KMAX) {
+    ++k;
+    p *= lam / (double)k;
+    F += p;
+  }
+  return k;
+}
+
+// draws of the cell pair (c, c + 1), c even: one Philox block serves both inversions
+__device__ __forceinline__ void poisson_pair(double l0, double l1, uint64_t c, uint32_t k0, uint32_t k1, int& n0,
+                                             int& n1) {
+  const Philox4 w = rng_block(c >> 1, 0, TAG_POISSON, k0, k1);
+  n0 = poisson_draw(l0, uniform_f64(w.x, w.y), c, k0, k1);
+  n1 = poisson_draw(l1, uniform_f64(w.z, w.w), c + 1, k0, k1);
 }
 
 // triangular in-cell offset of src/gauss_field.py:83-88 in cell units: R = 2u - 1,

@@ -3,14 +3,17 @@
 // src/populate_field.py:4-29) with the lognormal step of tests/create_lognormal.py:50-51
 // fused in, and an optional Zel'dovich redshift-space shift along z.
 //
-// Population is three streaming passes over the field, each of which recomputes the
-// per-cell rate instead of storing counts (the field is read three times; no n^3 count
-// array is written):
-//   sum   : FP64 per-block sums of rho = exp(bias * field) (or field)  -> fixed-order total
-//   count : Poisson counts per cell -> per-block totals -> exclusive scan (block offsets)
-//   place : counts again, block-local scan, one record (x, y, z) per particle
-// A block owns POP_CELLS consecutive cells in C order, visited in rounds of POP_NT
-// consecutive cells, so particles come out in cell order and every offset is a prefix sum.
+// Population passes:
+//   sum   : FP64 per-block sums of rho = exp(bias * field) (or field) -> fixed-order total
+//   count : Poisson counts per cell (one byte each, saturated) -> per-block totals ->
+//           exclusive scan (block offsets)
+//   mark  : block-local scans of the stored counts; each particle slot receives its cell
+//   emit  : one thread per particle: draws, offset, shift, wrap, the (x, y, z) record
+// A block owns POP_CELLS consecutive cells in C order, visited in rounds of 2 POP_NT
+// consecutive cells (a thread takes a pair, whose two Poisson draws share one Philox
+// block), so particles come out in cell order and every offset is a prefix sum.  Emitting
+// per particle rather than per cell keeps the random-number and FP64 work free of the
+// divergence of sparse cells (most cells hold no particle).
 #include "pk_common.h"
 #include "pk_internal.h"
 #include "pk_mock.h"
@@ -21,8 +24,8 @@ namespace {
 constexpr int AMP_NT = 256;
 constexpr int SPEC_NT = 256;
 constexpr int POP_NT = 256;
-constexpr int POP_PER = 16;
-constexpr int POP_CELLS = POP_NT * POP_PER;
+constexpr int POP_ROUNDS = 8;
+constexpr int POP_CELLS = 2 * POP_NT * POP_ROUNDS;
 constexpr int TOP_NT = 1024;
 constexpr int MISC_LEN = 4;   // [0] rate scale, [1] total (as int64 bits), [2] sum rho
 
@@ -36,7 +39,9 @@ size_t gauss_table_len(int n) { return (size_t)3 * (size_t)(n / 2) * (size_t)(n 
 
 size_t populate_ws_bytes(int n) {
   const size_t nb = (size_t)pop_blocks(n);
-  return align_up(nb * sizeof(double)) + align_up(nb * sizeof(long long)) + align_up(MISC_LEN * sizeof(double));
+  const size_t npair = ((size_t)n * n * n + 1) / 2;
+  return align_up(nb * sizeof(double)) + align_up(nb * sizeof(long long)) + align_up(MISC_LEN * sizeof(double)) +
+         align_up(npair * sizeof(uint16_t));
 }
 
 // ------------------------------------------------------------------------------------
@@ -98,6 +103,7 @@ struct PopWs {
   double* rsum;      // [nblk]
   long long* boff;   // [nblk] counts, then exclusive offsets
   double* misc;      // [MISC_LEN]
+  uint16_t* cnt;     // [ncell / 2] counts of cell pairs (two saturated bytes)
 };
 
 static PopWs carve_pop(void* ws, int n) {
@@ -109,6 +115,8 @@ static PopWs carve_pop(void* ws, int n) {
   w.boff = reinterpret_cast<long long*>(p);
   p += align_up(nb * sizeof(long long));
   w.misc = reinterpret_cast<double*>(p);
+  p += align_up(MISC_LEN * sizeof(double));
+  w.cnt = reinterpret_cast<uint16_t*>(p);
   return w;
 }
 
@@ -125,6 +133,7 @@ struct PopK {
   double* rsum;
   long long* boff;
   double* misc;
+  uint16_t* cnt;
   long long* total;  // caller's device int64 (nullable)
   float* pos;
   long long capacity;
@@ -134,10 +143,11 @@ __device__ __forceinline__ double cell_rho(float v, double bias) {
   return bias != 0.0 ? exp(bias * (double)v) : (double)v;
 }
 
-// cell of round k of this thread: a block owns POP_CELLS consecutive cells in C order,
-// visited in POP_PER rounds of POP_NT consecutive cells (one coalesced load per round)
-__device__ __forceinline__ int64_t pop_cell(int k) {
-  return (int64_t)blockIdx.x * POP_CELLS + (int64_t)k * POP_NT + threadIdx.x;
+// first cell of round k of this thread: a block owns POP_CELLS consecutive cells in C
+// order, visited in POP_ROUNDS rounds of 2 POP_NT cells; thread t takes the pair
+// (2t, 2t + 1) of each round
+__device__ __forceinline__ int64_t pop_pair(int k) {
+  return (int64_t)blockIdx.x * POP_CELLS + (int64_t)k * (2 * POP_NT) + 2 * (int64_t)threadIdx.x;
 }
 
 __device__ __forceinline__ long long block_sum_i64(long long v, long long* red) {
@@ -178,16 +188,20 @@ __device__ __forceinline__ long long block_scan_i64(long long v, long long* sh, 
 
 __global__ void __launch_bounds__(POP_NT) k_pop_sum(PopK a) {
   __shared__ double red[POP_NT / 64 + 1];
-  float v[POP_PER];
+  float v[2 * POP_ROUNDS];
 #pragma unroll
-  for (int k = 0; k < POP_PER; ++k) {
-    const int64_t c = pop_cell(k);
-    v[k] = c < a.ncell ? a.field[c] : 0.0f;
+  for (int k = 0; k < POP_ROUNDS; ++k) {
+    const int64_t c = pop_pair(k);
+    v[2 * k] = c < a.ncell ? a.field[c] : 0.0f;
+    v[2 * k + 1] = c + 1 < a.ncell ? a.field[c + 1] : 0.0f;
   }
   double s = 0.0;
 #pragma unroll
-  for (int k = 0; k < POP_PER; ++k)
-    if (pop_cell(k) < a.ncell) s += cell_rho(v[k], a.bias);
+  for (int k = 0; k < POP_ROUNDS; ++k) {
+    const int64_t c = pop_pair(k);
+    if (c < a.ncell) s += cell_rho(v[2 * k], a.bias);
+    if (c + 1 < a.ncell) s += cell_rho(v[2 * k + 1], a.bias);
+  }
   const double t = block_sum_f64_waves<POP_NT>(s, red);
   if (threadIdx.x == 0) a.rsum[blockIdx.x] = t;
 }
@@ -207,9 +221,11 @@ __global__ void __launch_bounds__(TOP_NT) k_pop_scale(PopK a) {
   }
 }
 
-__device__ __forceinline__ int cell_count(const PopK& a, int64_t c, double scale) {
-  if (c >= a.ncell) return 0;
-  return poisson_draw(cell_rho(a.field[c], a.bias) * scale, (uint64_t)c, a.k0, a.k1);
+// counts of the pair (c, c + 1); cells beyond the mesh count 0
+__device__ __forceinline__ void pair_counts(const PopK& a, int64_t c, double scale, int& n0, int& n1) {
+  const double l0 = c < a.ncell ? cell_rho(a.field[c], a.bias) * scale : 0.0;
+  const double l1 = c + 1 < a.ncell ? cell_rho(a.field[c + 1], a.bias) * scale : 0.0;
+  poisson_pair(l0, l1, (uint64_t)c, a.k0, a.k1, n0, n1);
 }
 
 __global__ void __launch_bounds__(POP_NT) k_pop_count(PopK a) {
@@ -217,7 +233,14 @@ __global__ void __launch_bounds__(POP_NT) k_pop_count(PopK a) {
   const double scale = a.misc[0];
   long long s = 0;
 #pragma unroll 1
-  for (int k = 0; k < POP_PER; ++k) s += cell_count(a, pop_cell(k), scale);
+  for (int k = 0; k < POP_ROUNDS; ++k) {
+    const int64_t c = pop_pair(k);
+    int n0, n1;
+    pair_counts(a, c, scale, n0, n1);
+    s += (long long)n0 + n1;
+    // stored saturated: 255 means "recount"
+    if (c < a.ncell) a.cnt[c >> 1] = (uint16_t)(min(n0, 255) | (min(n1, 255) << 8));
+  }
   const long long t = block_sum_i64(s, red);
   if (threadIdx.x == 0) a.boff[blockIdx.x] = t;
 }
@@ -244,39 +267,66 @@ __global__ void __launch_bounds__(TOP_NT) k_pop_scan(PopK a) {
   }
 }
 
-__global__ void __launch_bounds__(POP_NT) k_pop_place(PopK a) {
+// particle slots -> cell ids: the first word of each output record receives its cell
+// (as uint32), which k_pop_emit turns into the position
+__global__ void __launch_bounds__(POP_NT) k_pop_mark(PopK a) {
   __shared__ long long sh[2][POP_NT / 64];
-  const double scale = a.misc[0];
-  const int n = a.n;
-  const bool rsd = a.psi != nullptr && a.growth != 0.0;
   long long base = a.boff[blockIdx.x];
+  uint32_t* rec = reinterpret_cast<uint32_t*>(a.pos);
 #pragma unroll 1
-  for (int k = 0; k < POP_PER; ++k) {
-    const int64_t c = pop_cell(k);
-    const int cnt = cell_count(a, c, scale);
+  for (int k = 0; k < POP_ROUNDS; ++k) {
+    const int64_t c = pop_pair(k);
+    int n0 = 0, n1 = 0;
+    if (c < a.ncell) {
+      const uint16_t v = a.cnt[c >> 1];
+      n0 = v & 255;
+      n1 = v >> 8;
+      if (n0 == 255 || n1 == 255) pair_counts(a, c, a.misc[0], n0, n1);
+    }
     long long rtot;
     // two scan buffers: round k + 1 writes the other one, so one barrier per round
-    long long p = base + block_scan_i64<POP_NT>(cnt, sh[k & 1], rtot);
+    long long p = base + block_scan_i64<POP_NT>((long long)n0 + n1, sh[k & 1], rtot);
     base += rtot;
-    if (cnt == 0) continue;
-    const int iz = (int)(c % n);
-    const int iy = (int)((c / n) % n);
-    const int ix = (int)(c / ((int64_t)n * n));
-    const double shift = rsd ? a.growth * (double)a.psi[c] : 0.0;
-    for (int t = 0; t < cnt; ++t, ++p) {
+    for (int t = 0; t < n0 + n1; ++t, ++p) {
       if (p >= a.capacity) break;
-      const Philox4 w = rng_block((uint64_t)p, 0, TAG_DISP, a.k0, a.k1);
-      double r[3] = {((double)ix + 0.5 + tri_offset(w.x)) * a.H, ((double)iy + 0.5 + tri_offset(w.y)) * a.H,
-                     ((double)iz + 0.5 + tri_offset(w.z)) * a.H + shift};
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double u = r[d] - a.box * floor(r[d] / a.box);
-        float f = (float)u;
-        if (f >= (float)a.box) f = 0.0f;
-        a.pos[3 * p + d] = f;
-      }
+      rec[3 * p] = (uint32_t)(t < n0 ? c : c + 1);
     }
   }
+}
+
+// periodic wrap into [0, L) in FP64, rounded to float32 (a value that rounds up to L
+// becomes 0); values already inside skip the division
+__device__ __forceinline__ float wrap_box(double r, double box) {
+  double u = r;
+  if (!(u >= 0.0 && u < box)) {
+    u = r - box * floor(r / box);
+    if (u < 0.0) u += box;
+    if (u >= box) u -= box;
+  }
+  const float f = (float)u;
+  return f >= (float)box ? 0.0f : f;
+}
+
+// one thread per particle: cell centre + triangular offset (+ redshift-space shift),
+// wrapped into [0, L)
+__global__ void __launch_bounds__(POP_NT) k_pop_emit(PopK a) {
+  const long long p = (long long)blockIdx.x * POP_NT + threadIdx.x;
+  const long long total = reinterpret_cast<const long long*>(a.misc)[1];
+  if (p >= a.capacity || p >= total) return;
+  const uint32_t c = reinterpret_cast<const uint32_t*>(a.pos)[3 * p];
+  const uint32_t n = (uint32_t)a.n;
+  const uint32_t iz = c % n, cy = c / n;
+  const uint32_t iy = cy % n, ix = cy / n;
+  const Philox4 w = rng_block((uint64_t)p, 0, TAG_DISP, a.k0, a.k1);
+  const double shift = (a.psi != nullptr && a.growth != 0.0) ? a.growth * (double)a.psi[c] : 0.0;
+  double r[3] = {((double)ix + 0.5 + tri_offset(w.x)) * a.H, ((double)iy + 0.5 + tri_offset(w.y)) * a.H,
+                 ((double)iz + 0.5 + tri_offset(w.z)) * a.H + shift};
+  float f[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) f[d] = wrap_box(r[d], a.box);
+  a.pos[3 * p] = f[0];
+  a.pos[3 * p + 1] = f[1];
+  a.pos[3 * p + 2] = f[2];
 }
 
 namespace {
@@ -299,6 +349,7 @@ PopK make_pop(const float* field, const float* psi, int n, double box, double de
   a.rsum = w.rsum;
   a.boff = w.boff;
   a.misc = w.misc;
+  a.cnt = w.cnt;
   a.total = total;
   a.pos = pos;
   a.capacity = capacity;
@@ -319,8 +370,11 @@ int populate_count(const float* field, int n, double box, double density, double
 
 int populate_place(const float* field, const float* psi, double growth, int n, double box, double bias, uint64_t seed,
                    void* ws, float* pos, int64_t capacity, hipStream_t s) {
+  if (capacity == 0) return PK_OK;
   const PopK a = make_pop(field, psi, n, box, 0.0, bias, growth, seed, ws, nullptr, pos, (long long)capacity);
-  hipLaunchKernelGGL(k_pop_place, dim3(a.nblk), dim3(POP_NT), 0, s, a);
+  hipLaunchKernelGGL(k_pop_mark, dim3(a.nblk), dim3(POP_NT), 0, s, a);
+  const long long g = (capacity + POP_NT - 1) / POP_NT;
+  hipLaunchKernelGGL(k_pop_emit, dim3((unsigned)g), dim3(POP_NT), 0, s, a);
   return check_hip(hipGetLastError(), "population place launch");
 }
 

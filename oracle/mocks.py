@@ -28,8 +28,10 @@ algorithm around the draws is the reference's):
                               form of src/populate_field.py:57-69 is broken in the
                               reference); the k_z = n/2 plane is set to 0.
 * ``poisson``              <- inversion for lambda < 10, Hormann's PTRS transformed
-                              rejection (1993) for lambda >= 10; draws of cell c from
-                              Philox blocks (c, j, TAG_POISSON), j = 0, 1, ...
+                              rejection (1993) for lambda >= 10; the inversion uniform of
+                              cell c from the Philox block (c >> 1, 0, TAG_POISSON) (words
+                              0, 1 for even c, 2, 3 for odd c), rejection rounds from
+                              blocks (c, j, TAG_PTRS), j = 0, 1, ...
 * ``displacements``        <- src/gauss_field.py:83-88: R = 2u - 1,
                               sign(R) (1 - sqrt|R|) times the cell size; particle p draws
                               from the Philox block (p, 0, TAG_DISP).
@@ -64,6 +66,7 @@ PHILOX_W1 = 0xBB67AE85
 TAG_FIELD = 1
 TAG_POISSON = 2
 TAG_DISP = 3
+TAG_PTRS = 4
 
 INV_KMAX = 100        # inversion search bound (lambda < 10: the tail beyond is < 1e-60)
 PTRS_MAX_IT = 64      # rejection rounds before the deterministic fallback floor(lambda)
@@ -243,6 +246,27 @@ def log_factorial(k: float) -> float:
     return (x - 0.5) * float(np.log(x)) - x + 0.91893853320467274178 + corr
 
 
+def log_factorial_vec(k):
+    """log_factorial over an array of non-negative integer-valued floats."""
+    k = np.asarray(k, dtype=F64)
+    tab = np.array(_LOG_FACT_TAB)
+    x = k + 1.0
+    r = 1.0 / x
+    r2 = r * r
+    corr = r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0 - r2 * (1.0 / 1680.0))))
+    st = (x - 0.5) * np.log(x) - x + 0.91893853320467274178 + corr
+    return np.where(k < 16.0, tab[np.clip(k, 0, 15).astype(np.int64)], st)
+
+
+def poisson_uniform(cells, seed):
+    """Inversion uniform of cell c: block (c >> 1, 0, TAG_POISSON), words (x, y) for even c,
+    (z, w) for odd c."""
+    cells = np.asarray(cells, dtype=U64)
+    w0, w1, w2, w3 = block(cells >> U64(1), 0, TAG_POISSON, seed)
+    odd = (cells & U64(1)) == U64(1)
+    return np.where(odd, u52(w2, w3), u52(w0, w1))
+
+
 def poisson(lam, cells, seed):
     """Poisson draws for rates lam (FP64) of the given global cell indices."""
     lam = np.minimum(np.asarray(lam, dtype=F64), POISSON_LAM_MAX)
@@ -252,8 +276,7 @@ def poisson(lam, cells, seed):
     small = ok & (lam < POISSON_INV_MAX)
     if small.any():
         ls = lam[small]
-        w0, w1, _, _ = block(cells[small], 0, TAG_POISSON, seed)
-        u = u52(w0, w1)
+        u = poisson_uniform(cells[small], seed)
         p = np.exp(-ls)
         F = p.copy()
         k = np.zeros(ls.shape, dtype=np.int64)
@@ -264,26 +287,31 @@ def poisson(lam, cells, seed):
             F = np.where(act, F + p, F)
             act = act & (u > F) & (k < INV_KMAX)
         out[small] = k
-    big_idx = np.nonzero(ok & ~small)[0]
-    for i in big_idx:
-        L = float(lam[i])
+    big = np.nonzero(ok & ~small)[0]
+    if big.size:
+        L = lam[big]
+        cb = cells[big]
         loglam, a, b, invalpha, vr = _ptrs_consts(L)
-        res = None
+        res = np.floor(L)                     # fallback after PTRS_MAX_IT rounds
+        todo = np.arange(big.size)
         for j in range(PTRS_MAX_IT):
-            w0, w1, w2, w3 = block(cells[i], j, TAG_POISSON, seed)
-            U = float(u52(w0, w1)) - 0.5
-            V = float(u52(w2, w3))
-            us = 0.5 - abs(U)
-            kk = np.floor((2.0 * a / us + b) * U + L + 0.43)
-            if us >= 0.07 and V <= vr:
-                res = kk
+            if todo.size == 0:
                 break
-            if kk < 0 or (us < 0.013 and V > us):
-                continue
-            if np.log(V) + np.log(invalpha) - np.log(a / (us * us) + b) <= -L + kk * loglam - log_factorial(kk):
-                res = kk
-                break
-        out[i] = int(res) if res is not None else int(np.floor(L))
+            w0, w1, w2, w3 = block(cb[todo], j, TAG_PTRS, seed)
+            U = u52(w0, w1) - 0.5
+            V = u52(w2, w3)
+            us = 0.5 - np.abs(U)
+            Lt, at, bt = L[todo], a[todo], b[todo]
+            kk = np.floor((2.0 * at / us + bt) * U + Lt + 0.43)
+            acc1 = (us >= 0.07) & (V <= vr[todo])
+            rej = (kk < 0) | ((us < 0.013) & (V > us))
+            with np.errstate(invalid="ignore", divide="ignore"):
+                lhs = np.log(V) + np.log(invalpha[todo]) - np.log(at / (us * us) + bt)
+                rhs = -Lt + kk * loglam[todo] - log_factorial_vec(np.maximum(kk, 0.0))
+            acc = acc1 | (~rej & (lhs <= rhs))
+            res[todo[acc]] = kk[acc]
+            todo = todo[~acc]
+        out[big] = res.astype(np.int64)
     return out
 
 
@@ -317,7 +345,18 @@ def populate(field, n, box, density, seed, bias=0.0, psi_z=None, growth_rate=0.0
     pos = np.stack([(ix + 0.5 + d[:, 0]) * H, (iy + 0.5 + d[:, 1]) * H, (iz + 0.5 + d[:, 2]) * H], axis=1)
     if psi_z is not None and growth_rate != 0.0:
         pos[:, 2] += float(growth_rate) * np.asarray(psi_z, dtype=F32).reshape(-1)[cid].astype(F64)
-    pos = pos - box * np.floor(pos / box)
-    out = pos.astype(F32)
+    return wrap_box(pos, box), counts
+
+
+def wrap_box(r, box):
+    """Periodic wrap into [0, L) in FP64 (values inside are kept as they are), rounded to
+    float32; a value that rounds up to L becomes 0."""
+    r = np.asarray(r, dtype=F64)
+    inside = (r >= 0.0) & (r < box)
+    with np.errstate(invalid="ignore"):
+        u = r - box * np.floor(r / box)
+    u = np.where(u < 0.0, u + box, u)
+    u = np.where(u >= box, u - box, u)
+    out = np.where(inside, r, u).astype(F32)
     out[out >= F32(box)] = 0.0
-    return out, counts
+    return out

@@ -167,8 +167,7 @@ def test_poisson_inversion_is_the_quantile():
     lam = rng.random(20000) * 9.99
     cells = np.arange(lam.size, dtype=np.uint64) * 7 + 3
     k = M.poisson(lam, cells, 42)
-    w0, w1, _, _ = M.block(cells, 0, M.TAG_POISSON, 42)
-    u = M.u52(w0, w1)
+    u = M.poisson_uniform(cells, 42)
     q = stats.poisson.ppf(u, lam).astype(np.int64)
     # the inversion sums the pmf forward in FP64; scipy's quantile differs only where u is
     # within rounding of a CDF value
@@ -228,3 +227,23 @@ def test_log_factorial():
         want = math.lgamma(k + 1.0)
         got = M.log_factorial(float(k))
         assert abs(got - want) <= 2e-15 * max(1.0, abs(want)), (k, got, want)
+
+
+def test_wrap_box_stays_inside():
+    box = 1000.0
+    r = np.array([0.0, -0.0, 1e-300, -1e-300, box, box - 1e-13, -box, 2 * box - 1e-9, -3.5 * box, 123.25,
+                  np.nextafter(box, 0), np.nextafter(0.0, -1)])
+    w = M.wrap_box(r, box)
+    assert w.dtype == np.float32
+    assert np.all(w >= 0) and np.all(w < np.float32(box))
+    assert w[9] == np.float32(123.25)
+    # periodic images agree up to float32 rounding of the box
+    d = np.abs(w.astype(np.float64) - (r % box))
+    assert np.all(np.minimum(d, box - d) <= 1e-4)
+
+
+def test_log_factorial_vec_matches_scalar():
+    ks = np.array([0, 1, 5, 15, 16, 17, 40, 1000, 123456], dtype=np.float64)
+    v = M.log_factorial_vec(ks)
+    for k, x in zip(ks, v):
+        assert abs(x - M.log_factorial(float(k))) <= 1e-15 * max(1.0, abs(x))
