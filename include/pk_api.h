@@ -217,6 +217,28 @@ int pk_batch_run(pk_batch* batch, void* stream);
 int pk_batch_tile(pk_batch* batch, int* tile);
 int pk_batch_destroy(pk_batch* batch);
 
+/* --- configuration-space multipoles ----------------------------------------------------
+ * xi_vec(delta, box_size, s_edges) (src/correlations.py:119-187) and
+ * xi_vec_fundamental(delta, box_size) (:191-260): xi(s) = irfftn(|rfftn(delta) W|^2) binned
+ * by |s| over the full n^3 separation grid with Legendre weights of mu = s_z / |s|.
+ *   - bins: thresholds on m = sx^2 + sy^2 + sz^2 (signed grid separations) as for P_l, with
+ *     edges in grid units (s_edges n / L); modes[nb] = cells of the full grid per bin
+ *     (jax_powspec_amd.binning.bin_modes(thr, n, full=True)); 1 <= nb <= PK_XI_MAX_BINS;
+ *   - outputs: sums FP64 [nb][3] = sum over the bin's cells of xi_cell L_l; xi float32
+ *     [nb][3] = (2l+1) sums / (N n^3); nmodes float32 [nb];
+ *   - empty bins: xi 0 and nmodes +inf (xi_vec), or with PK_XI_EMPTY_NAN xi NaN and
+ *     nmodes 0 (xi_vec_fundamental); the bin holding s = 0 has NaN l = 2, 4 (mu = 0/0 in
+ *     the reference);
+ *   - the bin sums are accumulated in 64-bit fixed point (2^62 / (2 n^3 xi(0)) units),
+ *     so results are identical from run to run; workspace: pk_xi_workspace_size.
+ */
+#define PK_XI_MAX_BINS 1024
+#define PK_XI_EMPTY_NAN 4
+int pk_xi_workspace_size(pk_plan* plan, int nb, size_t* bytes);
+int pk_xi_mesh(pk_plan* plan, const float* mesh, const int32_t* thr, const int32_t* jstart, const int64_t* modes,
+               int nb, int window_p, int flags, double* sums, float* xi, float* nmodes, void* workspace,
+               size_t workspace_bytes, void* stream);
+
 /* --- seeded lognormal mocks -----------------------------------------------------------
  * The reference's mock pipeline (tests/create_lognormal.py:42-55): a Gaussian random
  * field from a tabulated P(k) (src/gauss_field.py:5-80), its inverse FFT

@@ -33,6 +33,8 @@ _f = C.c_float
 _d = C.c_double
 _sz = C.c_size_t
 _psz = C.POINTER(C.c_size_t)
+XI_MAX_BINS = 1024
+XI_EMPTY_NAN = 4
 FIELD_DELTA = 0
 FIELD_PSI_Z = 1
 BATCH_GRAPH = 1
@@ -76,6 +78,8 @@ SIGNATURES = [
     ("pk_batch_run", _i, [_vp, _vp]),
     ("pk_batch_tile", _i, [_vp, _vp]),
     ("pk_batch_destroy", _i, [_vp]),
+    ("pk_xi_workspace_size", _i, [_vp, _i, _psz]),
+    ("pk_xi_mesh", _i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     ("pk_gaussian_workspace_size", _i, [_i, _psz]),
     ("pk_gaussian_field", _i, [_vp, _d, _vp, _vp, _i, _i, C.c_uint64, _vp, _vp, _sz, _vp]),
     ("pk_c2r", _i, [_vp, _vp, _vp, _vp, _vp]),

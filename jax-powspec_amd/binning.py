@@ -109,11 +109,32 @@ def mode_multiplicity(n: int) -> np.ndarray:
     return mult
 
 
-def bin_modes(thr: np.ndarray, n: int):
+_MULT_FULL: dict = {}
+
+
+def full_grid_multiplicity(n: int) -> np.ndarray:
+    """mult[m] = number of cells of the full n^3 separation grid (signed coordinates, +n/2
+    kept) with sx^2 + sy^2 + sz^2 = m: the histogram population of the configuration-space
+    estimators (src/correlations.py:157-177); cached per n."""
+    hit = _MULT_FULL.get(n)
+    if hit is not None:
+        return hit
+    ki = np.arange(n)
+    ki = np.where(ki > n // 2, ki - n, ki).astype(np.int64)
+    hist_xy = np.bincount((ki[:, None] ** 2 + ki[None, :] ** 2).reshape(-1))
+    zc = np.bincount(ki * ki)                   # multiplicity of each sz^2
+    mult = np.zeros(max_m(n) + 1, np.int64)
+    for v in np.nonzero(zc)[0]:
+        mult[v:v + hist_xy.size] += zc[v] * hist_xy
+    _MULT_FULL[n] = mult
+    return mult
+
+
+def bin_modes(thr: np.ndarray, n: int, full: bool = False):
     """Exact per-bin mode counts (int64) and sums of |k| in units of k_F (FP64) over the
     stored half spectrum, from the integer thresholds: every entry counts once, the k = 0
-    entry included when its bin exists."""
-    mult = mode_multiplicity(n)
+    entry included when its bin exists.  full=True: over the full n^3 grid instead."""
+    mult = full_grid_multiplicity(n) if full else mode_multiplicity(n)
     nb = thr.size - 1
     m = np.arange(mult.size, dtype=np.int64)
     slot = slots_for_m(m, thr)
@@ -151,7 +172,8 @@ class TableCache:
     thresholds (e.g. the same k_F multiples from different box sizes) share one device
     table; both levels are LRUs."""
 
-    def __init__(self, capacity: int = 4096, tables: int = 256):
+    def __init__(self, capacity: int = 4096, tables: int = 256, full: bool = False):
+        self.full = full
         self.capacity = capacity
         self.tables = tables
         self._d: OrderedDict = OrderedDict()
@@ -168,7 +190,7 @@ class TableCache:
         tkey = (str(device), int(n), thr.tobytes())
         val = self._t.get(tkey)
         if val is None:
-            modes, ksum = bin_modes(thr, n)
+            modes, ksum = bin_modes(thr, n, self.full)
             val = BinTable(torch.from_numpy(thr).to(device), torch.from_numpy(jst).to(device), thr.size - 1,
                            torch.from_numpy(modes).to(device), torch.from_numpy(ksum).to(device))
             self._t[tkey] = val
@@ -183,3 +205,5 @@ class TableCache:
 
 
 TABLES = TableCache()
+# tables over the full separation grid (configuration-space multipoles)
+TABLES_FULL = TableCache(full=True)

@@ -28,7 +28,8 @@ import torch
 
 from . import _device, _lib, binning
 
-__all__ = ["powspec_vec", "powspec_vec_fundamental", "estimate", "rfftn", "Estimator", "WINDOW_P"]
+__all__ = ["powspec_vec", "powspec_vec_fundamental", "xi_vec", "xi_vec_fundamental", "xi_vec_coords",
+           "s_edges_conv", "estimate", "rfftn", "Estimator", "WINDOW_P"]
 
 WINDOW_P = {"NONE": 0, "NGP": 1, "CIC": 2, "CIC_REF": 2, "TSC": 3}
 
@@ -117,6 +118,74 @@ def powspec_vec_fundamental(delta, box_size, *, mas="CIC", shot_noise=0.0, full=
     k3D = (out.kmean * kF).to(torch.float32)
     res = (k3D, out.pk, out.nmodes)
     return res + (_extras(out, kF),) if full else res
+
+
+def _xi(delta, edges_grid, p, flags):
+    device = _device.device_of(delta)
+    mesh = _device.as_mesh(delta, device)
+    n = mesh.shape[0]
+    plan = _device.plan(n, device)
+    tab = binning.TABLES_FULL.get(device, n, edges_grid)
+    nb = tab.nb
+    if nb > _lib.XI_MAX_BINS:
+        raise ValueError(f"xi supports at most {_lib.XI_MAX_BINS} bins (got {nb})")
+    lib = _lib.load()
+    nbytes = C.c_size_t()
+    with torch.cuda.device(device):
+        _lib.check("pk_xi_workspace_size", lib.pk_xi_workspace_size(plan.handle, nb, C.byref(nbytes)))
+        ws = _device.workspace(nbytes.value, device)
+        sums = torch.empty((nb, 3), dtype=torch.float64, device=device)
+        xi = torch.empty((nb, 3), dtype=torch.float32, device=device)
+        nm = torch.empty(nb, dtype=torch.float32, device=device)
+        _lib.check("pk_xi_mesh", lib.pk_xi_mesh(
+            plan.handle, _device.ptr(mesh), _device.ptr(tab.thr), _device.ptr(tab.jst), _device.ptr(tab.modes), nb,
+            p, flags, _device.ptr(sums), _device.ptr(xi), _device.ptr(nm), _device.ptr(ws), nbytes.value,
+            _device.stream_ptr(device)))
+    return xi, nm, sums, tab, device
+
+
+def s_edges_conv(dims, box_size, s_edges):
+    """k-space form of separation edges, kF s n / L (src/correlations.py:269-272)."""
+    kF = 2.0 * np.pi / float(box_size)
+    return kF * np.asarray(s_edges, np.float64) * int(dims) / float(box_size)
+
+
+def xi_vec_coords(dims, box_size, k_edges):
+    """Bin centres of xi_vec for k-space edges (src/correlations.py:262-266)."""
+    kF = 2.0 * np.pi / float(box_size)
+    ke = np.asarray(k_edges, np.float64) / kF
+    return 0.5 * (ke[1:] + ke[:-1]) * (float(box_size) / int(dims))
+
+
+def xi_vec(delta, box_size, s_edges, *, mas="CIC", full=False):
+    """Correlation-function multipoles xi_l(s), l = 0, 2, 4 (src/correlations.py:119-187).
+
+    xi(s) = irfftn(|rfftn(delta) W|^2) over the full n^3 separation grid, binned by |s|
+    with the reference's histogram rules (edges s_edges n / L in grid units), mu = s_z/|s|.
+    Returns (r3D, xi3D[nb, 3] float32, Nmodes3D float32 with inf in empty bins, where xi is
+    0).  The bin holding s = 0 (an edge at or below 0) has NaN l = 2, 4, as the reference.
+    """
+    if hasattr(s_edges, "detach"):
+        s_edges = s_edges.detach().cpu().double().numpy()
+    n = int(delta.shape[0])
+    e = np.asarray(s_edges, np.float64).reshape(-1) * n / float(box_size)
+    xi, nm, sums, tab, device = _xi(delta, e, _window_p(mas), 0)
+    r = torch.as_tensor((0.5 * (e[1:] + e[:-1]) * (float(box_size) / n)).astype(np.float32), device=device)
+    res = (r, xi, nm)
+    return res + ({"sums": sums, "counts": tab.modes},) if full else res
+
+
+def xi_vec_fundamental(delta, box_size, *, mas="CIC", full=False):
+    """xi_l(s) in grid-spacing-wide shells [j, j+1), j = 1..kmax (src/correlations.py:191-260):
+    r3D is the cell-averaged |s| of each shell; empty shells give NaN."""
+    n = int(delta.shape[0])
+    e = binning.fundamental_grid_edges(n)
+    xi, nm, sums, tab, device = _xi(delta, e, _window_p(mas), _lib.XI_EMPTY_NAN)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        modes = tab.modes.cpu().numpy().astype(np.float64)
+        r = tab.ksum.cpu().numpy() / modes * (float(box_size) / n)
+    res = (torch.as_tensor(r.astype(np.float32), device=device), xi, nm)
+    return res + ({"sums": sums, "counts": tab.modes},) if full else res
 
 
 def rfftn(delta, *, contrast_stats=None):

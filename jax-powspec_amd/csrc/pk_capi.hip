@@ -984,3 +984,64 @@ int pk_populate_place(const float* field, const float* psi_z, double growth_rate
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// configuration-space multipoles
+// ------------------------------------------------------------------------------------
+namespace {
+size_t xi_ws_bytes(int n, int nb) {
+  const size_t half = (size_t)(n / 2 + 1) * n * n * sizeof(float2);
+  return align_up(half) + align_up((size_t)xi_grid_x(n) * sizeof(double)) + align_up(2 * sizeof(double)) +
+         align_up((size_t)nb * 3 * sizeof(unsigned long long)) + align_up((size_t)nb * sizeof(int));
+}
+}  // namespace
+
+extern "C" {
+
+int pk_xi_workspace_size(pk_plan* p, int nb, size_t* bytes) {
+  int e = check_plan(p);
+  if (e) return e;
+  if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
+  if (nb < 1 || nb > PK_XI_MAX_BINS) return set_error(PK_E_INVALID, "xi needs 1..%d bins (nb=%d)", PK_XI_MAX_BINS, nb);
+  *bytes = xi_ws_bytes(p->n, nb);
+  return PK_OK;
+}
+
+int pk_xi_mesh(pk_plan* p, const float* mesh, const int32_t* thr, const int32_t* jstart, const int64_t* modes, int nb,
+               int window_p, int flags, double* sums, float* xi, float* nmodes, void* workspace,
+               size_t workspace_bytes, void* stream) {
+  int e = check_plan(p);
+  if (e) return e;
+  const int n = p->n;
+  if (!mesh || !thr || !jstart || !modes || !sums || !xi || !nmodes)
+    return set_error(PK_E_INVALID, "null mesh / table / output pointer");
+  if (nb < 1 || nb > PK_XI_MAX_BINS) return set_error(PK_E_INVALID, "xi needs 1..%d bins (nb=%d)", PK_XI_MAX_BINS, nb);
+  if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
+  const size_t need = xi_ws_bytes(n, nb);
+  if (!workspace || workspace_bytes < need)
+    return set_error(PK_E_WORKSPACE, "xi workspace %zu < %zu bytes", workspace_bytes, need);
+  char* w = static_cast<char*>(workspace);
+  float2* spec = reinterpret_cast<float2*>(w);
+  w += align_up((size_t)(n / 2 + 1) * n * n * sizeof(float2));
+  XiArgs a{};
+  a.xi0_part = reinterpret_cast<double*>(w);
+  a.gx = xi_grid_x(n);
+  w += align_up((size_t)a.gx * sizeof(double));
+  a.scale = reinterpret_cast<double*>(w);
+  w += align_up(2 * sizeof(double));
+  a.fixed = reinterpret_cast<unsigned long long*>(w);
+  w += align_up((size_t)nb * 3 * sizeof(unsigned long long));
+  a.nanflag = reinterpret_cast<int*>(w);
+  a.thr = thr;
+  a.jst = jstart;
+  a.modes = modes;
+  a.nb = nb;
+  a.jlen = isqrt_c(3 * (n / 2) * (n / 2)) + 2;
+  a.empty_nan = (flags & PK_XI_EMPTY_NAN) ? 1 : 0;
+  a.sums = sums;
+  a.xi = xi;
+  a.nmodes = nmodes;
+  return fft_xi(n, mesh, spec, p->tw, p->wtab + (size_t)window_p * n, a, as_stream(stream));
+}
+
+}  // extern "C"

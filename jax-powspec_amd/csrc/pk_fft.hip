@@ -762,6 +762,227 @@ k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, co
 }
 
 // ------------------------------------------------------------------------------------
+// configuration-space multipoles xi_l(s) (src/correlations.py:119-260):
+//   x pass: forward DFT along x, |delta_k|^2 W_k times irfftn's 1/n^3 (2/n^3 on interior
+//           kz planes), inverse DFT along kx; FP64 partials of their sum = xi(s = 0),
+//           the largest value of the separation grid (|xi(s)| <= xi(0) for a spectrum >= 0)
+//   y pass: inverse (k_ypass_inverse)
+//   z pass: C2R rows binned by |s| on the fly (the n^3 separation grid is never stored):
+//           each thread walks a run of z, sums in FP64 while the bin stays the same and
+//           adds each run to 64-bit fixed-point LDS bins (integer sums do not depend on
+//           order, so results are run-to-run identical); blocks add their bins to global
+//           accumulators; S = 2^e with (n^3 2 xi(0)) S < 2^62 bounds every bin sum
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTX)
+k_xpass_power_inverse(float2* __restrict__ spec, const float2* __restrict__ twg, const float* __restrict__ wt,
+                      double* __restrict__ xi0_part) {
+  using Cf = FFTCfg<N>;
+  constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = (N / 2 + 1) * YT;
+  constexpr int H = N / 2;
+  __shared__ float2 buf[N * PS];
+  __shared__ float2 tw[N];
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  const double s_edge = 1.0 / ((double)N * (double)N * (double)N);
+  double acc = 0.0;
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int kz = tile / YT, yt = tile % YT;
+    const float sc = (float)((kz == 0 || kz == H) ? s_edge : 2.0 * s_edge);
+    float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
+    stage_tile<N, C, PS, NT>(buf, base, N);
+    __syncthreads();
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+    const float wz = wt[kz];
+    for (int i = tid; i < N * C; i += NT) {
+      const int c = i % C, r = i / C;
+      const float2 v = buf[r * PS + c];
+      const float pw = (v.x * v.x + v.y * v.y) * (wt[r] * wt[yt * C + c] * wz) * sc;
+      acc += (double)pw;
+      buf[r * PS + c] = make_float2(pw, 0.0f);   // real: its conjugate is itself
+    }
+    __syncthreads();
+    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+    for (int i = tid; i < N * C; i += NT) {
+      const float2 v = buf[(i / C) * PS + (i % C)];
+      base[(size_t)(i / C) * N + (i % C)] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+  }
+  const double t = block_sum_f64(acc, red, NT);
+  if (tid == 0) xi0_part[blockIdx.x] = t;
+}
+
+// fixed-order xi(0) -> scale; clears the accumulators of this estimate
+__global__ void __launch_bounds__(256) k_xi_scale(XiArgs a, int n) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < a.gx; i += 256) s += a.xi0_part[i];
+  const double xi0 = block_sum_f64(s, red, 256);
+  for (int i = threadIdx.x; i < 3 * a.nb; i += 256) a.fixed[i] = 0ull;
+  for (int i = threadIdx.x; i < a.nb; i += 256) a.nanflag[i] = 0;
+  if (threadIdx.x == 0) {
+    const double bound = 2.0 * xi0 * (double)n * (double)n * (double)n;
+    int e = 0;
+    double S = 1.0;
+    if (bound > 0.0 && bound < 1e300) {
+      (void)frexp(bound, &e);          // bound < 2^e
+      S = ldexp(1.0, 62 - e);
+    }
+    a.scale[0] = S;
+    a.scale[1] = 1.0 / S;
+  }
+}
+
+template <int N> struct XiBinCfg {
+  static constexpr int NC = N / 2, C = FFTCfg<N>::CZ, NT = FFTCfg<N>::NTZ;
+  // complex values (two z each) per run: the largest divisor of NC not above NC C / NT
+  static constexpr int want = cmax(1, NC * C / NT);
+  static constexpr int div_le(int v, int d) { return (v % d == 0) ? d : div_le(v, d - 1); }
+  static constexpr int CH = div_le(NC, want);
+  static constexpr int RUNS = C * (NC / CH);
+};
+
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTZ)
+k_zpass_c2r_bin(const float2* __restrict__ bspec, const float2* __restrict__ twg, XiArgs a) {
+  using Cf = FFTCfg<N>;
+  using BC = XiBinCfg<N>;
+  constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = N * YT;
+  constexpr int H = N / 2;
+  constexpr int CH = BC::CH;
+  __shared__ float2 buf[C * CS];
+  __shared__ float2 tw[N];
+  __shared__ unsigned long long bins[3 * XI_MAX_BINS];
+  __shared__ int thr_s[XI_MAX_BINS + 1];
+  __shared__ int jst_s[isqrt_c(3 * H * H) + 2];
+  const int tid = threadIdx.x;
+  const int nb = a.nb;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int i = tid; i < 3 * nb; i += NT) bins[i] = 0ull;
+  for (int i = tid; i <= nb; i += NT) thr_s[i] = a.thr[i];
+  for (int i = tid; i < a.jlen; i += NT) jst_s[i] = a.jst[i];
+  const double S = a.scale[0];
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int x = tile / YT, yt = tile % YT;
+    {
+      constexpr int TOT = (NC + 1) * C, PER = (TOT + NT - 1) / NT;
+      float2 v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) v[k] = bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)];
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) buf[(i % C) * CS + i / C] = v[k];
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < (NC / 2 + 1) * C; i += NT) {
+      const int c = i % C, k = i / C;
+      const int kk = NC - k;
+      float2* row = buf + c * CS;
+      auto Y = [&](int q) {
+        const float2 b = row[q];
+        if (q == 0 || q == NC) return make_float2(b.x, 0.0f);
+        return make_float2(0.5f * b.x, 0.5f * b.y);
+      };
+      const float2 yk = Y(k), ykk = Y(kk);
+      auto zval = [&](const float2 p, const float2 q2, int q) {
+        const float2 sm = make_float2(p.x + q2.x, p.y - q2.y);
+        const float2 d = make_float2(p.x - q2.x, p.y + q2.y);
+        const float2 e = make_float2(tw[q].x, -tw[q].y);
+        const float2 ed = cmul(e, d);
+        return make_float2(sm.x - ed.y, sm.y + ed.x);
+      };
+      const float2 zk = zval(yk, ykk, k);
+      const float2 zkk = zval(ykk, yk, kk);
+      row[k] = make_float2(zk.x, -zk.y);
+      if (kk != k && kk < NC) row[kk] = make_float2(zkk.x, -zkk.y);
+    }
+    __syncthreads();
+    lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
+    const int sx = x <= H ? x : x - N;
+    for (int w = tid; w < BC::RUNS; w += NT) {
+      const int c = w / (NC / CH), j0 = (w % (NC / CH)) * CH;
+      const int y = yt * C + c;
+      const int sy = y <= H ? y : y - N;
+      const int mxy = sx * sx + sy * sy;
+      int cur = -1;
+      double a0 = 0.0, a2 = 0.0, a4 = 0.0;
+      auto flush = [&]() {
+        if (cur >= 0) {
+          atomicAdd(&bins[3 * cur], (unsigned long long)__double2ll_rn(a0 * S));
+          atomicAdd(&bins[3 * cur + 1], (unsigned long long)__double2ll_rn(a2 * S));
+          atomicAdd(&bins[3 * cur + 2], (unsigned long long)__double2ll_rn(a4 * S));
+        }
+        a0 = a2 = a4 = 0.0;
+      };
+      for (int q = 0; q < CH; ++q) {
+        const float2 v = buf[c * CS + j0 + q];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int z = 2 * (j0 + q) + e;
+          const float val = e ? -v.y : v.x;      // (u(2j), u(2j+1)) = conj of the FFT output
+          const int sz = z <= H ? z : z - N;
+          const int z2 = sz * sz;
+          const int m = mxy + z2;
+          if (!(cur >= 0 && thr_s[cur] <= m && m < thr_s[cur + 1])) {
+            flush();
+            cur = find_bin(m, thr_s, jst_s, nb);
+          }
+          if (cur < 0) continue;
+          const double dv = (double)val;
+          a0 += dv;
+          if (m == 0) {
+            a.nanflag[cur] = 1;     // the s = 0 cell: mu = 0 / 0 (src/correlations.py:163)
+          } else {
+            const float rm = 1.0f / (float)m;
+            const float l2 = 0.5f * (float)(3 * z2 - m) * rm;
+            const double num4 = (double)z2 * (35.0 * (double)z2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
+            const float l4 = 0.125f * (float)num4 * rm * rm;
+            a2 += dv * (double)l2;
+            a4 += dv * (double)l4;
+          }
+        }
+      }
+      flush();
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < 3 * nb; i += NT)
+    if (bins[i]) atomicAdd(&a.fixed[i], bins[i]);
+}
+
+// FP64 sums and xi_l = (2l+1) sum_l / (N n^3); empty bins per the variant
+__global__ void __launch_bounds__(256) k_xi_finalize(XiArgs a, int n) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= a.nb) return;
+  const double inv = a.scale[1];
+  const double n3 = (double)n * (double)n * (double)n;
+  const long long cnt = a.modes[b];
+  const bool nan24 = a.nanflag[b] != 0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const double sl = (double)(long long)a.fixed[3 * b + l] * inv;
+    a.sums[3 * b + l] = sl;
+    float v;
+    if (cnt == 0) v = a.empty_nan ? __int_as_float(0x7fc00000) : 0.0f;
+    else if (l > 0 && nan24) v = __int_as_float(0x7fc00000);
+    else v = (float)(sl / (double)cnt * (double)(4 * l + 1) / n3);
+    a.xi[3 * b + l] = v;
+  }
+  a.nmodes[b] = cnt == 0 ? (a.empty_nan ? 0.0f : __int_as_float(0x7f800000)) : (float)cnt;
+}
+
+// ------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------
 template <typename K>
@@ -885,6 +1106,26 @@ template <int N> struct FFTLaunch {
     hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw);
     hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw, mesh);
     return check_hip(hipGetLastError(), "inverse FFT launch");
+  }
+  static int xi_gx() {
+    static int gx = 0;
+    if (!gx) gx = persistent_grid(k_xpass_power_inverse<N>, Cf::NTX, XTILES);
+    return gx;
+  }
+  static int xi(const float* mesh, float2* spec, const float2* tw, const float* wt, const XiArgs& a,
+                hipStream_t s) {
+    static int gy = 0, gz = 0;
+    if (!gy) gy = persistent_grid(k_ypass_inverse<N>, Cf::NTY, YTILES);
+    if (!gz) gz = persistent_grid(k_zpass_c2r_bin<N>, Cf::NTZ, ZTILES);
+    int e = zpass(mesh, spec, tw, nullptr, nullptr, s);
+    if (!e) e = ypass(spec, tw, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_xpass_power_inverse<N>, dim3(xi_gx()), dim3(Cf::NTX), 0, s, spec, tw, wt, a.xi0_part);
+    hipLaunchKernelGGL(k_xi_scale, dim3(1), dim3(256), 0, s, a, N);
+    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, spec, tw);
+    hipLaunchKernelGGL(k_zpass_c2r_bin<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)spec, tw, a);
+    hipLaunchKernelGGL(k_xi_finalize, dim3((a.nb + 255) / 256), dim3(256), 0, s, a, N);
+    return check_hip(hipGetLastError(), "xi launch");
   }
   static int xpass_bin_jobs(const Job* jobs, int njobs, int max_nb, int gx, const float2* tw, hipStream_t s) {
     StageTimer t(ST_XPASS_BIN, s);
@@ -1021,6 +1262,25 @@ int fft_c2r(int n, int src, const float2* spec_in, float2* scratch, const FieldG
             hipStream_t s) {
   switch (n) {
 #define PK_CASE(V) case V: return FFTLaunch<V>::c2r(src, spec_in, scratch, g, tw, mesh, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int xi_grid_x(int n) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::xi_gx();
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return 0;
+}
+
+int fft_xi(int n, const float* mesh, float2* spec, const float2* tw, const float* wt, const XiArgs& a,
+           hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::xi(mesh, spec, tw, wt, a, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }

@@ -180,6 +180,32 @@ int populate_count(const float* field, int n, double box, double density, double
 int populate_place(const float* field, const float* psi, double growth, int n, double box, double bias, uint64_t seed,
                    void* ws, float* pos, int64_t capacity, hipStream_t s);
 
+// ---- configuration-space multipoles (pk_fft.hip) -------------------------------------------
+// largest bin count of the separation binning (its 64-bit accumulators live in LDS)
+constexpr int XI_MAX_BINS = 1024;   // = PK_XI_MAX_BINS
+struct XiArgs {
+  const int* thr;                 // [nb + 1] integer thresholds on m = sx^2 + sy^2 + sz^2
+  const int* jst;                 // isqrt start table
+  const int64_t* modes;           // [nb] cells of the full grid per bin
+  int nb;
+  int jlen;                       // entries of jst
+  int empty_nan;                  // empty bins: NaN and count 0 (else 0 and count inf)
+  double* xi0_part;               // [Gx] FP64 partials of xi(0) (x pass)
+  int gx;
+  double* scale;                  // [2] fixed-point scale S = 2^e and 1 / S
+  unsigned long long* fixed;      // [nb][3] fixed-point bin sums (two's complement)
+  int* nanflag;                   // [nb] bin holds s = 0 (mu undefined: l = 2, 4 NaN)
+  double* sums;                   // [nb][3] FP64 bin sums (output)
+  float* xi;                      // [nb][3] (output)
+  float* nmodes;                  // [nb] (output)
+};
+int xi_grid_x(int n);
+// mesh -> forward z, y passes -> x pass |delta_k|^2 W + inverse -> inverse y -> C2R rows
+// binned by separation -> finalize; spec is a half-spectrum buffer, wt the window row of
+// the plan (exponent p)
+int fft_xi(int n, const float* mesh, float2* spec, const float2* tw, const float* wt, const XiArgs& a,
+           hipStream_t s);
+
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
 }  // namespace pk

@@ -19,6 +19,15 @@ This is a NumPy FP64 restatement of the reference's algorithm, function by funct
 * ``powspec_vec_fundamental``    <- src/correlations.py:60-117 (k output: mode-averaged k,
   which deviates from the reference's ``.set`` at :100).
 * ``subtract_shot_noise``        <- tests/correlations.py:36,83 (monopole only).
+* ``xi_grid``                    <- src/correlations.py:134-155: irfftn of
+  |rfftn(delta) * correction|^2 (the separation grid).
+* ``xi_vec``                     <- src/correlations.py:119-187: histogram of |s| (grid
+  units) over the full n^3 grid with edges s_edges n / L, mu = s_z / |s| (NaN at s = 0),
+  Nmodes -> inf in empty bins (xi 0 there), xi_l = (2l+1) sum / N / n^3, r = edge midpoints
+  times L / n.  kedges = s_edges n / L in FP64 (the reference forms kF s n / L / kF in
+  float32).
+* ``xi_vec_fundamental``         <- src/correlations.py:191-260: slots int(|s|) = isqrt(m),
+  slot 0 dropped, r = mean |s| times L / n, 0 / 0 = NaN in empty slots.
 
 How each function is pinned (independently of this file) is listed in tests/test_oracle.py.
 The jax/jaxlib arithmetic the reference delegates to (XLA scatter, DUCC FFT, jnp.histogram)
@@ -540,3 +549,83 @@ def estimate_backward(x, y, z, w, box_size, n, k_edges, grad_pk, stencil="CIC_RE
     grad_pos = S * inv * w[:, None] * ga
     grad_w = S * gw
     return grad_pos, grad_w
+
+
+# ------------------------------------------------------------------------------------
+# configuration-space multipoles (src/correlations.py:119-260)
+# ------------------------------------------------------------------------------------
+def real_grids(n):
+    """Signed separations (ki of src/correlations.py:139-141) on the full n^3 grid."""
+    ki = wavenumbers(n)
+    return ki[:, None, None], ki[None, :, None], ki[None, None, :]
+
+
+def xi_grid(delta, p=2):
+    """irfftn(|rfftn(delta) correction|^2) in FP64 (src/correlations.py:134-155)."""
+    delta = np.asarray(delta, F64)
+    n = delta.shape[0]
+    dk = np.fft.rfftn(delta, axes=(0, 1, 2)) * window_correction(n, p)
+    return np.fft.irfftn((dk * dk.conj()).real, s=(n, n, n), axes=(0, 1, 2))
+
+
+def _xi_sums(g, slot, nb, m, sz2):
+    sel = slot >= 0
+    s = slot[sel]
+    v = g.reshape(-1)[sel]
+    mm = m[sel]
+    z2 = sz2[sel]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        L2 = (3 * z2 - mm).astype(F64) / (2.0 * mm)             # NaN at s = 0 (mu = 0/0)
+        L4 = (35 * z2 * z2 - 30 * z2 * mm + 3 * mm * mm).astype(F64) / (8.0 * mm.astype(F64) ** 2)
+    S = np.stack([np.bincount(s, weights=v * w, minlength=nb)[:nb] for w in (1.0, L2, L4)], axis=1)
+    with np.errstate(invalid="ignore"):
+        A = np.stack([np.bincount(s, weights=np.abs(v * w), minlength=nb)[:nb] for w in (1.0, L2, L4)], axis=1)
+    cnt = np.bincount(s, minlength=nb)[:nb].astype(np.int64)
+    ksum = np.bincount(s, weights=np.sqrt(mm.astype(F64)), minlength=nb)[:nb]
+    return S, A, cnt, ksum
+
+
+def _xi_full_grid(n):
+    sx, sy, sz = real_grids(n)
+    m = (sx * sx + sy * sy + sz * sz).astype(np.int64)
+    sz2 = np.broadcast_to(sz * sz, m.shape).astype(np.int64)
+    return m.reshape(-1), sz2.reshape(-1)
+
+
+def xi_vec(delta, box_size, s_edges, p=2):
+    """FP64 restatement of src/correlations.py:119-187.
+
+    Returns (r3D, xi3D[nb, 3], Nmodes3D, extra): Nmodes3D is inf in empty bins, where xi is
+    0; extra holds int64 counts and the same-bin absolute scale A[nb, 3] =
+    (2l+1) sum |xi_cell L_l| / (N n^3) used as the parity tolerance unit."""
+    n = np.asarray(delta).shape[0]
+    g = xi_grid(delta, p)
+    m, sz2 = _xi_full_grid(n)
+    kedges = np.asarray(s_edges, F64) * n / float(box_size)
+    slot = jnp_histogram_slots(np.sqrt(m.astype(F64)), kedges)
+    nb = kedges.size - 1
+    S, A, cnt, _ = _xi_sums(g, slot, nb, m, sz2)
+    N = np.where(cnt == 0, np.inf, cnt.astype(F64))
+    f = np.array([1.0, 5.0, 9.0]) / float(n) ** 3
+    xi = S / N[:, None] * f
+    r = 0.5 * (kedges[1:] + kedges[:-1]) * (float(box_size) / n)
+    with np.errstate(invalid="ignore"):
+        scale = A / N[:, None] * f
+    return r, xi, N, {"counts": cnt, "scale": scale}
+
+
+def xi_vec_fundamental(delta, box_size, p=2):
+    """FP64 restatement of src/correlations.py:191-260 (slots j = isqrt(m), j >= 1)."""
+    n = np.asarray(delta).shape[0]
+    g = xi_grid(delta, p)
+    m, sz2 = _xi_full_grid(n)
+    kmax = int(np.sqrt(3 * (n // 2) ** 2))
+    j = np.array([int(v) for v in np.sqrt(np.arange(m.max() + 1, dtype=F64))])[m]   # int32(|s|)
+    slot = np.where(j >= 1, j - 1, -1)
+    S, A, cnt, ksum = _xi_sums(g, slot, kmax, m, sz2)
+    f = np.array([1.0, 5.0, 9.0]) / float(n) ** 3
+    with np.errstate(invalid="ignore", divide="ignore"):
+        xi = S / cnt[:, None] * f
+        r = ksum / cnt * (float(box_size) / n)
+        scale = A / cnt[:, None] * f
+    return r, xi, cnt.astype(F64), {"counts": cnt, "scale": scale}
