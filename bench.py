@@ -15,6 +15,7 @@ baseline (FP32 NumPy/SciPy restatement of the reference algorithm on this host).
 from __future__ import annotations
 
 import argparse
+import math
 import json
 import os
 import sys
@@ -497,6 +498,104 @@ def bench_mock(dev, reps=5, cpu=True):
     return out
 
 
+def xi_bytes(n):
+    """Minimum HBM bytes of one xi_vec: z pass (read mesh, write half spectrum), forward y
+    (read + write), x pass with power and inverse (read + write), inverse y (read + write),
+    C2R with binning (read half spectrum)."""
+    half = 8 * n * n * (n // 2 + 1)
+    return 4 * n ** 3 + half + 2 * half + 2 * half + 2 * half + half
+
+
+def bench_xi(dev, reps=10, cpu=True):
+    """xi_vec (src/correlations.py:119-187) of a 512^3 Gaussian field with the separation
+    bins of tests/create_lognormal.py:66 (s_edges = arange(1e-3, 200, 5), L = 1500)."""
+    import torch
+    from jax_powspec_amd import correlations, gauss_field as GF
+
+    n, box = N_MESH, BOX
+    edges = np.arange(1e-3, 200.0, 5.0)
+    d = GF.gaussian_mesh(n, MOCK_KF, MOCK_PKF, 1, 21, box)
+    correlations.xi_vec(d, box, edges)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r, xi, nm = correlations.xi_vec(d, box, edges)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ms = float(np.median(ts)) * 1e3
+    gbs = xi_bytes(n) / (ms * 1e-3) / 1e9
+    out = {"workload": f"xi_vec {n}^3, L={box:g}, s_edges=arange(1e-3, 200, 5) ({edges.size - 1} bins)",
+           "ms": round(ms, 3), "per_s": round(1e3 / ms, 2), "algorithmic_gbs": round(gbs, 1),
+           "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+    del d
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import ref_fp64 as R
+        ns = 128
+        dh = np.random.default_rng(0).standard_normal((ns, ns, ns)).astype(np.float32)
+        t0 = time.perf_counter()
+        R.xi_vec(dh, box * ns / n, edges * ns / n)
+        dt = time.perf_counter() - t0
+        scaled = dt * (n / ns) ** 3 * math.log2(n) / math.log2(ns)
+        out["cpu_baseline"] = {"ms_scaled": round(scaled * 1e3, 1), "unit": "ms per xi_vec", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle FP64 NumPy xi_vec at {ns}^3 ({dt:.2f} s), scaled by n^3 log n "
+                                         f"to {n}^3"}
+        out["gpu_over_cpu"] = round(scaled * 1e3 / ms, 1)
+    return out
+
+
+def bench_bispec(dev, reps=3, cpu=True):
+    """bispec (src/correlations.py:334-458) of a 512^3 lognormal field with the triangles of
+    tests/bispec.py:53-54 (k1 = 0.1, k2 = 0.2, 20 angles; L = 1500)."""
+    import torch
+    from jax_powspec_amd import correlations, gauss_field as GF
+
+    n, box = N_MESH, BOX
+    k1, k2 = 0.1, 0.2
+    theta = np.linspace(0, np.pi, 20)
+    g = GF.gaussian_mesh(n, MOCK_KF, MOCK_PKF, 1, 31, box)
+    d = torch.exp(0.5 * g / g.std())
+    d = d / d.mean() - 1.0
+    del g
+    correlations.bispec(d, box, k1, k2, theta)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        correlations.bispec(d, box, k1, k2, theta)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ms = float(np.median(ts)) * 1e3
+    half = 8 * n * n * (n // 2 + 1)
+    mesh = 4 * n ** 3
+    nsh = theta.size + 2
+    # forward transform; per shell two inverse x + y passes and a C2R (two stored for the first
+    # two shells, product-mesh reads for the rest); the pair products
+    byts = (mesh + 5 * half) + nsh * 2 * (4 * half) + 4 * (half + mesh) + (nsh - 2) * 2 * (half + mesh) + 6 * mesh
+    gbs = byts / (ms * 1e-3) / 1e9
+    out = {"workload": f"bispec {n}^3, L={box:g}, k1={k1}, k2={k2}, {theta.size} angles ({nsh} shells)",
+           "ms": round(ms, 2), "per_s": round(1e3 / ms, 2), "algorithmic_gbs": round(gbs, 1),
+           "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+    del d
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import ref_fp64 as R
+        ns = 128
+        dh = np.random.default_rng(0).standard_normal((ns, ns, ns)).astype(np.float32)
+        t0 = time.perf_counter()
+        R.bispec(dh, box * ns / n, k1, k2, theta)
+        dt = time.perf_counter() - t0
+        scaled = dt * (n / ns) ** 3 * math.log2(n) / math.log2(ns)
+        out["cpu_baseline"] = {"ms_scaled": round(scaled * 1e3, 1), "unit": "ms per bispec", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle FP64 NumPy bispec at {ns}^3 ({dt:.2f} s), scaled by n^3 log n "
+                                         f"to {n}^3"}
+        out["gpu_over_cpu"] = round(scaled * 1e3 / ms, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,6 +612,10 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the 128^3 / 1e6-particle plain case")
     ap.add_argument("--only-served", action="store_true", help="run only the serving benchmark")
     ap.add_argument("--no-mock", action="store_true", help="skip the mock-generator measurement")
+    ap.add_argument("--no-xi", action="store_true", help="skip the configuration-space measurement")
+    ap.add_argument("--no-bispec", action="store_true", help="skip the bispectrum measurement")
+    ap.add_argument("--only-bispec", action="store_true", help="run only the bispectrum measurement")
+    ap.add_argument("--only-xi", action="store_true", help="run only the configuration-space measurement")
     ap.add_argument("--only-mock", action="store_true", help="run only the mock-generator measurement")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the calibrated HBM traffic summary of two rocprofv3 --pmc runs and exit")
@@ -551,6 +654,12 @@ def main():
         return
     if args.only_served:
         print(json.dumps({"served": bench_served(dev)}), flush=True)
+        return
+    if args.only_bispec:
+        print(json.dumps({"bispec": bench_bispec(dev, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.only_xi:
+        print(json.dumps({"xi": bench_xi(dev, cpu=not args.no_cpu)}), flush=True)
         return
     if args.only_mock:
         print(json.dumps({"mock": bench_mock(dev, cpu=not args.no_cpu)}), flush=True)
@@ -635,6 +744,16 @@ def main():
         mock = bench_mock(dev, cpu=not args.no_cpu)
         log(f"mock: {mock}")
         torch.cuda.empty_cache()
+    xi = None
+    if not args.no_xi:
+        xi = bench_xi(dev, cpu=not args.no_cpu)
+        log(f"xi: {xi}")
+        torch.cuda.empty_cache()
+    bis = None
+    if not args.no_bispec:
+        bis = bench_bispec(dev, cpu=not args.no_cpu)
+        log(f"bispec: {bis}")
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -660,6 +779,8 @@ def main():
         "batched": batched,
         "served": served,
         "mock": mock,
+        "xi": xi,
+        "bispec": bis,
     }
     print(json.dumps(out), flush=True)
 

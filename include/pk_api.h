@@ -239,6 +239,24 @@ int pk_xi_mesh(pk_plan* plan, const float* mesh, const int32_t* thr, const int32
                int nb, int window_p, int flags, double* sums, float* xi, float* nmodes, void* workspace,
                size_t workspace_bytes, void* stream);
 
+/* --- bispectrum -------------------------------------------------------------------------
+ * bispec(delta, box_size, k1, k2, theta) (src/correlations.py:334-458): for each k-shell i
+ * (k1, k2, then k3(theta)), delta_i = irfftn(shell delta_k W) and I_i = irfftn(shell);
+ * P_i = sum delta_i^2 / sum I_i^2 (L/n^2)^3, and for i >= 2
+ * B = sum delta1 delta2 delta_i / sum I1 I2 I_i (L^2/n^3)^3, Q = B / (P1P2 + P1Pi + P2Pi).
+ *   - shell_lo / shell_hi: HOST int32[nshell], shell i = {lo_i <= m < hi_i} on
+ *     m = kx^2 + ky^2 + kz^2 (jax_powspec_amd.correlations derives them from
+ *     k_i / kF -/+ 1 in FP64); 2 <= nshell <= PK_BISPEC_MAX_SHELLS;
+ *   - window_p: the reference multiplies delta_k by prod (x / sin x)^2, i.e. p = 1;
+ *   - outputs: sums FP64 [nshell][4] (sum delta^2, sum I^2, triple sums), pk float32
+ *     [nshell], bk and qk float32 [nshell - 2]; workspace: pk_bispec_workspace_size.
+ */
+#define PK_BISPEC_MAX_SHELLS 1024
+int pk_bispec_workspace_size(pk_plan* plan, int nshell, size_t* bytes);
+int pk_bispec_mesh(pk_plan* plan, const float* mesh, const int32_t* shell_lo, const int32_t* shell_hi, int nshell,
+                   int window_p, double box_size, double* sums, float* pk, float* bk, float* qk, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
 /* --- seeded lognormal mocks -----------------------------------------------------------
  * The reference's mock pipeline (tests/create_lognormal.py:42-55): a Gaussian random
  * field from a tabulated P(k) (src/gauss_field.py:5-80), its inverse FFT

@@ -34,6 +34,7 @@ _d = C.c_double
 _sz = C.c_size_t
 _psz = C.POINTER(C.c_size_t)
 XI_MAX_BINS = 1024
+BISPEC_MAX_SHELLS = 1024
 XI_EMPTY_NAN = 4
 FIELD_DELTA = 0
 FIELD_PSI_Z = 1
@@ -78,6 +79,8 @@ SIGNATURES = [
     ("pk_batch_run", _i, [_vp, _vp]),
     ("pk_batch_tile", _i, [_vp, _vp]),
     ("pk_batch_destroy", _i, [_vp]),
+    ("pk_bispec_workspace_size", _i, [_vp, _i, _psz]),
+    ("pk_bispec_mesh", _i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     ("pk_xi_workspace_size", _i, [_vp, _i, _psz]),
     ("pk_xi_mesh", _i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     ("pk_gaussian_workspace_size", _i, [_i, _psz]),

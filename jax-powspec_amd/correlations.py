@@ -29,7 +29,7 @@ import torch
 from . import _device, _lib, binning
 
 __all__ = ["powspec_vec", "powspec_vec_fundamental", "xi_vec", "xi_vec_fundamental", "xi_vec_coords",
-           "s_edges_conv", "estimate", "rfftn", "Estimator", "WINDOW_P"]
+           "s_edges_conv", "bispec", "estimate", "rfftn", "Estimator", "WINDOW_P"]
 
 WINDOW_P = {"NONE": 0, "NGP": 1, "CIC": 2, "CIC_REF": 2, "TSC": 3}
 
@@ -186,6 +186,56 @@ def xi_vec_fundamental(delta, box_size, *, mas="CIC", full=False):
         r = tab.ksum.cpu().numpy() / modes * (float(box_size) / n)
     res = (torch.as_tensor(r.astype(np.float32), device=device), xi, nm)
     return res + ({"sums": sums, "counts": tab.modes},) if full else res
+
+
+def bispec_shells(n, box_size, k1, k2, theta):
+    """k_all = (k1, k2, k3(theta)) (src/correlations.py:352-356) and the integer shells
+    lo <= m < hi equivalent to k_i/kF - 1 <= |k| < k_i/kF + 1 (:358-359, :377) in FP64."""
+    kF = 2.0 * np.pi / float(box_size)
+    theta = np.asarray(theta.detach().cpu() if hasattr(theta, "detach") else theta, np.float64).reshape(-1)
+    k3 = np.sqrt((k2 * np.sin(theta)) ** 2 + (k2 * np.cos(theta) + k1) ** 2)
+    k_all = np.concatenate([[float(k1), float(k2)], k3])
+    never = binning.max_m(n) + 1
+    lo = binning._min_m_ge((k_all - kF) / kF, never, strict=False)
+    hi = binning._min_m_ge((k_all + kF) / kF, never, strict=False)
+    return k_all, theta, lo.astype(np.int32), hi.astype(np.int32)
+
+
+def bispec(delta, box_size, k1, k2, theta, *, mas="CIC", full=False):
+    """Bispectrum and reduced bispectrum of (k1, k2, theta) triangles (src/correlations.py:
+    334-458).  Returns (k_all[bins+2], Pk[bins+2], theta, B[bins], Q[bins]) as float32
+    device tensors (k_all and theta from the host computation).  mas: 'CIC' (the
+    reference's (x / sin x)^2 per axis on delta_k) or 'NONE'."""
+    key = "NONE" if mas is None else str(mas).upper()
+    if key not in ("CIC", "CIC_REF", "NONE"):
+        raise ValueError(f"bispec supports mas='CIC' or 'NONE', got {mas!r}")
+    p = 0 if key == "NONE" else 1
+    device = _device.device_of(delta)
+    mesh = _device.as_mesh(delta, device)
+    n = mesh.shape[0]
+    plan = _device.plan(n, device)
+    k_all, th, lo, hi = bispec_shells(n, box_size, k1, k2, theta)
+    ns = k_all.size
+    if ns > _lib.BISPEC_MAX_SHELLS:
+        raise ValueError(f"at most {_lib.BISPEC_MAX_SHELLS - 2} theta bins")
+    lib = _lib.load()
+    nbytes = C.c_size_t()
+    with torch.cuda.device(device):
+        _lib.check("pk_bispec_workspace_size", lib.pk_bispec_workspace_size(plan.handle, ns, C.byref(nbytes)))
+        ws = _device.workspace(nbytes.value, device)
+        sums = torch.empty((ns, 4), dtype=torch.float64, device=device)
+        pk = torch.empty(ns, dtype=torch.float32, device=device)
+        bk = torch.empty(max(ns - 2, 1), dtype=torch.float32, device=device)
+        qk = torch.empty(max(ns - 2, 1), dtype=torch.float32, device=device)
+        lo_h = (C.c_int32 * ns)(*lo.tolist())
+        hi_h = (C.c_int32 * ns)(*hi.tolist())
+        _lib.check("pk_bispec_mesh", lib.pk_bispec_mesh(
+            plan.handle, _device.ptr(mesh), C.cast(lo_h, C.c_void_p), C.cast(hi_h, C.c_void_p), ns, p,
+            float(box_size), _device.ptr(sums), _device.ptr(pk), _device.ptr(bk), _device.ptr(qk), _device.ptr(ws),
+            nbytes.value, _device.stream_ptr(device)))
+    f32 = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=device)
+    res = (f32(k_all), pk, f32(th), bk[:ns - 2], qk[:ns - 2])
+    return res + ({"sums": sums, "shell_lo": lo, "shell_hi": hi},) if full else res
 
 
 def rfftn(delta, *, contrast_stats=None):

@@ -1045,3 +1045,63 @@ int pk_xi_mesh(pk_plan* p, const float* mesh, const int32_t* thr, const int32_t*
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// bispectrum
+// ------------------------------------------------------------------------------------
+namespace {
+size_t bispec_ws_bytes(int n, int nshell) {
+  const size_t half = align_up((size_t)(n / 2 + 1) * n * n * sizeof(float2));
+  const size_t mesh = align_up((size_t)n * n * n * sizeof(float));
+  return 2 * half + 4 * mesh + align_up((size_t)nshell * BK_SLOTS * BK_PART_STRIDE * sizeof(double));
+}
+}  // namespace
+
+extern "C" {
+
+int pk_bispec_workspace_size(pk_plan* p, int nshell, size_t* bytes) {
+  int e = check_plan(p);
+  if (e) return e;
+  if (!bytes) return set_error(PK_E_INVALID, "null bytes pointer");
+  if (nshell < 2 || nshell > PK_BISPEC_MAX_SHELLS)
+    return set_error(PK_E_INVALID, "bispectrum needs 2..%d shells (got %d)", PK_BISPEC_MAX_SHELLS, nshell);
+  *bytes = bispec_ws_bytes(p->n, nshell);
+  return PK_OK;
+}
+
+int pk_bispec_mesh(pk_plan* p, const float* mesh, const int32_t* shell_lo, const int32_t* shell_hi, int nshell,
+                   int window_p, double box_size, double* sums, float* pk, float* bk, float* qk, void* workspace,
+                   size_t workspace_bytes, void* stream) {
+  int e = check_plan(p);
+  if (e) return e;
+  const int n = p->n;
+  if (!mesh || !shell_lo || !shell_hi || !sums || !pk) return set_error(PK_E_INVALID, "null mesh / shell / output");
+  if (nshell < 2 || nshell > PK_BISPEC_MAX_SHELLS)
+    return set_error(PK_E_INVALID, "bispectrum needs 2..%d shells (got %d)", PK_BISPEC_MAX_SHELLS, nshell);
+  if (nshell > 2 && (!bk || !qk)) return set_error(PK_E_INVALID, "null B / Q output");
+  if (window_p < 0 || window_p > 3) return set_error(PK_E_INVALID, "window exponent p=%d not in 0..3", window_p);
+  if (!(box_size > 0.0)) return set_error(PK_E_INVALID, "box_size must be > 0");
+  const size_t need = bispec_ws_bytes(n, nshell);
+  if (!workspace || workspace_bytes < need)
+    return set_error(PK_E_WORKSPACE, "bispectrum workspace %zu < %zu bytes", workspace_bytes, need);
+  std::vector<int> lo(shell_lo, shell_lo + nshell), hi(shell_hi, shell_hi + nshell);
+  char* w = static_cast<char*>(workspace);
+  const size_t half = align_up((size_t)(n / 2 + 1) * n * n * sizeof(float2));
+  const size_t msz = align_up((size_t)n * n * n * sizeof(float));
+  float2* spec = reinterpret_cast<float2*>(w);
+  float2* scratch = reinterpret_cast<float2*>(w + half);
+  float* meshes[4];
+  for (int i = 0; i < 4; ++i) meshes[i] = reinterpret_cast<float*>(w + 2 * half + i * msz);
+  BispecArgs a{};
+  a.nshell = nshell;
+  a.box = box_size;
+  a.part = reinterpret_cast<double*>(w + 2 * half + 4 * msz);
+  a.sums = sums;
+  a.pk = pk;
+  a.bk = bk;
+  a.qk = qk;
+  return fft_bispec(n, mesh, spec, scratch, meshes, p->tw, p->wtab + (size_t)window_p * n, lo.data(), hi.data(), a,
+                    as_stream(stream));
+}
+
+}  // extern "C"

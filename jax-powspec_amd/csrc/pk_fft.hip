@@ -714,7 +714,7 @@ k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, fl
 template <int N, int SRC>
 __global__ void __launch_bounds__(FFTCfg<N>::NTX)
 k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, const float2* __restrict__ twg,
-                FieldGen g) {
+                FieldGen g, ShellMask sm) {
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
   constexpr int YT = N / C;
@@ -743,6 +743,28 @@ k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, co
         const int i = tid + k * NT;
         if (TOT % NT == 0 || i < TOT) buf[(i / C) * PS + (i % C)] = make_float2(sc * v[k].x, -sc * v[k].y);
       }
+    } else if constexpr (SRC == C2R_SHELL || SRC == C2R_SHELL_ONE) {
+      // the k-shell lo <= m < hi of the stored spectrum times the window (src/correlations.py:
+      // 373-382), or the shell's indicator
+      const float2* src = spec_in + (size_t)kz * N * N + (size_t)yt * C;
+      const float wz = sm.wt[kz];
+      for (int i = tid; i < N * C; i += NT) {
+        const int r = i / C, c = i % C;
+        const int y = yt * C + c;
+        const int sx = r <= H ? r : r - N, sy = y <= H ? y : y - N;
+        const int m = sx * sx + sy * sy + kz * kz;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (m >= sm.lo && m < sm.hi) {
+          if constexpr (SRC == C2R_SHELL) {
+            const float2 t = src[(size_t)r * N + c];
+            const float w = sm.wt[r] * sm.wt[y] * wz * sc;
+            v = make_float2(w * t.x, -w * t.y);
+          } else {
+            v = make_float2(sc, 0.0f);
+          }
+        }
+        buf[r * PS + c] = v;
+      }
     } else {
       for (int i = tid; i < N * C; i += NT) {
         const int r = i / C, c = i % C;
@@ -759,6 +781,144 @@ k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, co
     }
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------------------------------------
+// bispectrum (src/correlations.py:334-458): per k-shell the inverse transforms of the
+// shell's delta_k W and of its indicator; the first two shells are stored and multiplied
+// (delta1 delta2, I1 I2), later shells are reduced as their C2R rows are produced:
+// sum v^2 and sum v prod (FP64 per thread, fixed-order block sums), never stored.
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTZ)
+k_zpass_c2r_reduce(const float2* __restrict__ bspec, const float2* __restrict__ twg, const float* __restrict__ prod,
+                   double* __restrict__ part_sq, double* __restrict__ part_prod) {
+  using Cf = FFTCfg<N>;
+  constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
+  constexpr int YT = N / C;
+  constexpr int NTILES = N * YT;
+  __shared__ float2 buf[C * CS];
+  __shared__ float2 tw[N];
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  double ssq = 0.0, spr = 0.0;
+  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+    const int x = tile / YT, yt = tile % YT;
+    {
+      constexpr int TOT = (NC + 1) * C, PER = (TOT + NT - 1) / NT;
+      float2 v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) v[k] = bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)];
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (TOT % NT == 0 || i < TOT) buf[(i % C) * CS + i / C] = v[k];
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < (NC / 2 + 1) * C; i += NT) {
+      const int c = i % C, k = i / C;
+      const int kk = NC - k;
+      float2* row = buf + c * CS;
+      auto Y = [&](int q) {
+        const float2 b = row[q];
+        if (q == 0 || q == NC) return make_float2(b.x, 0.0f);
+        return make_float2(0.5f * b.x, 0.5f * b.y);
+      };
+      const float2 yk = Y(k), ykk = Y(kk);
+      auto zval = [&](const float2 p, const float2 q2, int q) {
+        const float2 sm = make_float2(p.x + q2.x, p.y - q2.y);
+        const float2 d = make_float2(p.x - q2.x, p.y + q2.y);
+        const float2 e = make_float2(tw[q].x, -tw[q].y);
+        const float2 ed = cmul(e, d);
+        return make_float2(sm.x - ed.y, sm.y + ed.x);
+      };
+      const float2 zk = zval(yk, ykk, k);
+      const float2 zkk = zval(ykk, yk, kk);
+      row[k] = make_float2(zk.x, -zk.y);
+      if (kk != k && kk < NC) row[kk] = make_float2(zkk.x, -zkk.y);
+    }
+    __syncthreads();
+    lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
+    const float2* prow = reinterpret_cast<const float2*>(prod + ((size_t)x * N + (size_t)yt * C) * N);
+    for (int i = tid; i < C * NC; i += NT) {
+      const int c = i / NC, jj = i % NC;
+      const float2 v = buf[c * CS + jj];
+      const float u0 = v.x, u1 = -v.y;              // (u(2j), u(2j+1))
+      const float2 p = prow[(size_t)c * NC + jj];
+      ssq += (double)u0 * (double)u0 + (double)u1 * (double)u1;
+      spr += (double)u0 * (double)p.x + (double)u1 * (double)p.y;
+    }
+    __syncthreads();
+  }
+  const double a = block_sum_f64(ssq, red, NT);
+  const double b = block_sum_f64(spr, red, NT);
+  if (tid == 0) {
+    part_sq[blockIdx.x] = a;
+    part_prod[blockIdx.x] = b;
+  }
+}
+
+// products of the first two shells' fields in place (p <- p q) and sums of their squares
+constexpr int PAIR_NT = 256;
+__global__ void __launch_bounds__(PAIR_NT) k_bispec_pair(float* __restrict__ p, const float* __restrict__ q,
+                                                         int64_t total, double* __restrict__ part_p,
+                                                         double* __restrict__ part_q) {
+  __shared__ double red[PAIR_NT];
+  double sp = 0.0, sq = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PAIR_NT + threadIdx.x; i < total; i += (int64_t)gridDim.x * PAIR_NT) {
+    const float a = p[i], b = q[i];
+    sp += (double)a * (double)a;
+    sq += (double)b * (double)b;
+    p[i] = a * b;
+  }
+  const double ta = block_sum_f64(sp, red, PAIR_NT);
+  const double tb = block_sum_f64(sq, red, PAIR_NT);
+  if (threadIdx.x == 0) {
+    part_p[blockIdx.x] = ta;
+    part_q[blockIdx.x] = tb;
+  }
+}
+
+// P_i = sum delta_i^2 / sum I_i^2 (L/n^2)^3, B = sum delta1 delta2 delta3 / sum I1 I2 I3
+// (L^2/n^3)^3, Q = B / (P1 P2 + P1 P3 + P2 P3) (src/correlations.py:392-452)
+__global__ void __launch_bounds__(64) k_bispec_finalize(BispecArgs a, int n, int g_pair, int g_red) {
+  const int sh = blockIdx.x;
+  if (sh >= a.nshell) return;
+  double v[BK_SLOTS];
+  for (int t = 0; t < BK_SLOTS; ++t) {
+    const int cnt = (sh < 2) ? (t < 2 ? g_pair : 0) : g_red;
+    const double* pp = a.part + ((size_t)sh * BK_SLOTS + t) * BK_PART_STRIDE;
+    double s = 0.0;
+    if (threadIdx.x == 0)
+      for (int i = 0; i < cnt; ++i) s += pp[i];
+    v[t] = s;
+  }
+  if (threadIdx.x != 0) return;
+  for (int t = 0; t < BK_SLOTS; ++t) a.sums[sh * BK_SLOTS + t] = v[t];
+  const double h = a.box / ((double)n * (double)n);
+  const double norm_p = h * h * h;
+  a.pk[sh] = (float)(v[0] / v[1] * norm_p);
+}
+
+__global__ void __launch_bounds__(64) k_bispec_bq(BispecArgs a, int n) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i + 2 >= a.nshell) return;
+  const double h = a.box / ((double)n * (double)n);
+  const double norm_p = h * h * h;
+  const double g = a.box * a.box / ((double)n * (double)n * (double)n);
+  const double norm_b = g * g * g;
+  const double* S = a.sums;
+  const double p1 = S[0] / S[1] * norm_p, p2 = S[BK_SLOTS] / S[BK_SLOTS + 1] * norm_p;
+  const double* s3 = S + (size_t)(i + 2) * BK_SLOTS;
+  const double p3 = s3[0] / s3[1] * norm_p;
+  const double b = s3[2] / s3[3] * norm_b;
+  a.bk[i] = (float)b;
+  a.qk[i] = (float)(b / (p1 * p2 + p1 * p3 + p2 * p3));
 }
 
 // ------------------------------------------------------------------------------------
@@ -858,7 +1018,7 @@ k_zpass_c2r_bin(const float2* __restrict__ bspec, const float2* __restrict__ twg
   constexpr int CH = BC::CH;
   __shared__ float2 buf[C * CS];
   __shared__ float2 tw[N];
-  __shared__ unsigned long long bins[3 * XI_MAX_BINS];
+  extern __shared__ unsigned long long bins[];   // [3 nb]: dynamic LDS sized at launch
   __shared__ int thr_s[XI_MAX_BINS + 1];
   __shared__ int jst_s[isqrt_c(3 * H * H) + 2];
   const int tid = threadIdx.x;
@@ -915,6 +1075,15 @@ k_zpass_c2r_bin(const float2* __restrict__ bspec, const float2* __restrict__ twg
       const int y = yt * C + c;
       const int sy = y <= H ? y : y - N;
       const int mxy = sx * sx + sy * sy;
+      // the run's m range (|s_z| is unimodal in z: largest at z = n/2): runs whose every
+      // m is outside [thr[0], thr[nb]) are skipped without touching their values
+      {
+        const int zf = 2 * j0, zl = 2 * (j0 + CH) - 1;
+        const int af = zf <= H ? zf : N - zf, al = zl <= H ? zl : N - zl;
+        const int amin = j0 == 0 ? 0 : min(af, al);
+        const int amax = (zf <= H && zl >= H) ? H : max(af, al);
+        if (mxy + amin * amin >= thr_s[nb] || mxy + amax * amax < thr_s[0]) continue;
+      }
       int cur = -1;
       double a0 = 0.0, a2 = 0.0, a4 = 0.0;
       auto flush = [&]() {
@@ -944,7 +1113,7 @@ k_zpass_c2r_bin(const float2* __restrict__ bspec, const float2* __restrict__ twg
           if (m == 0) {
             a.nanflag[cur] = 1;     // the s = 0 cell: mu = 0 / 0 (src/correlations.py:163)
           } else {
-            const float rm = 1.0f / (float)m;
+            const float rm = __builtin_amdgcn_rcpf((float)m);
             const float l2 = 0.5f * (float)(3 * z2 - m) * rm;
             const double num4 = (double)z2 * (35.0 * (double)z2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
             const float l4 = 0.125f * (float)num4 * rm * rm;
@@ -1088,11 +1257,74 @@ template <int N> struct FFTLaunch {
     return check_hip(hipGetLastError(), "adjoint FFT launch");
   }
   template <int SRC>
-  static int xinv(const float2* in, float2* scratch, const float2* tw, const FieldGen& g, hipStream_t s) {
+  static int xinv(const float2* in, float2* scratch, const float2* tw, const FieldGen& g, hipStream_t s,
+                  ShellMask sm = ShellMask{0, 0, nullptr}) {
     static int gx = 0;
     if (!gx) gx = persistent_grid(k_xpass_inverse<N, SRC>, Cf::NTX, XTILES);
-    hipLaunchKernelGGL((k_xpass_inverse<N, SRC>), dim3(gx), dim3(Cf::NTX), 0, s, in, scratch, tw, g);
+    hipLaunchKernelGGL((k_xpass_inverse<N, SRC>), dim3(gx), dim3(Cf::NTX), 0, s, in, scratch, tw, g, sm);
     return check_hip(hipGetLastError(), "inverse x pass launch");
+  }
+  static int red_grid() {
+    static int g = 0;
+    if (!g) g = cmin(BK_PART_STRIDE, persistent_grid(k_zpass_c2r_reduce<N>, Cf::NTZ, ZTILES));
+    return g;
+  }
+  static int pair_grid() {
+    static int g = 0;
+    if (!g) {
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      g = cmin(BK_PART_STRIDE, num_cus(dev) * 8);
+    }
+    return g;
+  }
+  // shell inverse: x pass (masked source) + inverse y, into scratch
+  static int shell_inverse(bool one, const float2* spec, float2* scratch, const float2* tw, ShellMask sm,
+                           hipStream_t s) {
+    static int gy = 0;
+    if (!gy) gy = persistent_grid(k_ypass_inverse<N>, Cf::NTY, YTILES);
+    FieldGen g{};
+    g.n = N;
+    int e = one ? xinv<C2R_SHELL_ONE>(spec, scratch, tw, g, s, sm) : xinv<C2R_SHELL>(spec, scratch, tw, g, s, sm);
+    if (e) return e;
+    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw);
+    return check_hip(hipGetLastError(), "shell inverse launch");
+  }
+  static int bispec(const float* mesh, float2* spec, float2* scratch, float* const* meshes, const float2* tw,
+                    const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s) {
+    static int gz = 0;
+    if (!gz) gz = persistent_grid(k_zpass_c2r<N>, Cf::NTZ, ZTILES);
+    int e = zpass(mesh, spec, tw, nullptr, nullptr, s);
+    if (!e) e = ypass(spec, tw, s);
+    if (!e) e = xpass(spec, tw, s);
+    if (e) return e;
+    // first two shells: stored fields
+    for (int sh = 0; sh < 2; ++sh)
+      for (int one = 0; one < 2; ++one) {
+        e = shell_inverse(one != 0, spec, scratch, tw, ShellMask{lo[sh], hi[sh], wt}, s);
+        if (e) return e;
+        hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,
+                           meshes[one * 2 + sh]);
+      }
+    const int gp = pair_grid();
+    const int64_t total = (int64_t)N * N * N;
+    auto slot = [&](int sh, int t) { return a.part + ((size_t)sh * BK_SLOTS + t) * BK_PART_STRIDE; };
+    hipLaunchKernelGGL(k_bispec_pair, dim3(gp), dim3(PAIR_NT), 0, s, meshes[0], (const float*)meshes[1], total,
+                       slot(0, 0), slot(1, 0));
+    hipLaunchKernelGGL(k_bispec_pair, dim3(gp), dim3(PAIR_NT), 0, s, meshes[2], (const float*)meshes[3], total,
+                       slot(0, 1), slot(1, 1));
+    const int gr = red_grid();
+    for (int sh = 2; sh < a.nshell; ++sh)
+      for (int one = 0; one < 2; ++one) {
+        e = shell_inverse(one != 0, spec, scratch, tw, ShellMask{lo[sh], hi[sh], wt}, s);
+        if (e) return e;
+        hipLaunchKernelGGL(k_zpass_c2r_reduce<N>, dim3(gr), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,
+                           (const float*)meshes[one * 2], slot(sh, one), slot(sh, 2 + one));
+      }
+    hipLaunchKernelGGL(k_bispec_finalize, dim3(a.nshell), dim3(64), 0, s, a, N, gp, gr);
+    if (a.nshell > 2)
+      hipLaunchKernelGGL(k_bispec_bq, dim3((a.nshell - 2 + 63) / 64), dim3(64), 0, s, a, N);
+    return check_hip(hipGetLastError(), "bispectrum launch");
   }
   static int c2r(int src, const float2* in, float2* scratch, const FieldGen& g, const float2* tw, float* mesh,
                  hipStream_t s) {
@@ -1116,14 +1348,15 @@ template <int N> struct FFTLaunch {
                 hipStream_t s) {
     static int gy = 0, gz = 0;
     if (!gy) gy = persistent_grid(k_ypass_inverse<N>, Cf::NTY, YTILES);
-    if (!gz) gz = persistent_grid(k_zpass_c2r_bin<N>, Cf::NTZ, ZTILES);
+    if (!gz) gz = persistent_grid(k_zpass_c2r_bin<N>, Cf::NTZ, ZTILES);   // sized without dynamic LDS
     int e = zpass(mesh, spec, tw, nullptr, nullptr, s);
     if (!e) e = ypass(spec, tw, s);
     if (e) return e;
     hipLaunchKernelGGL(k_xpass_power_inverse<N>, dim3(xi_gx()), dim3(Cf::NTX), 0, s, spec, tw, wt, a.xi0_part);
     hipLaunchKernelGGL(k_xi_scale, dim3(1), dim3(256), 0, s, a, N);
     hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, spec, tw);
-    hipLaunchKernelGGL(k_zpass_c2r_bin<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)spec, tw, a);
+    hipLaunchKernelGGL(k_zpass_c2r_bin<N>, dim3(gz), dim3(Cf::NTZ), (size_t)a.nb * 3 * sizeof(unsigned long long), s,
+                       (const float2*)spec, tw, a);
     hipLaunchKernelGGL(k_xi_finalize, dim3((a.nb + 255) / 256), dim3(256), 0, s, a, N);
     return check_hip(hipGetLastError(), "xi launch");
   }
@@ -1275,6 +1508,16 @@ int xi_grid_x(int n) {
 #undef PK_CASE
   }
   return 0;
+}
+
+int fft_bispec(int n, const float* mesh, float2* spec, float2* scratch, float* const* meshes, const float2* tw,
+               const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::bispec(mesh, spec, scratch, meshes, tw, wt, lo, hi, a, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
 }
 
 int fft_xi(int n, const float* mesh, float2* spec, const float2* tw, const float* wt, const XiArgs& a,

@@ -158,8 +158,14 @@ struct FieldGen {
   int rayleigh;
   float l2pi;   // L / (2 pi): physical units of the displacement component
 };
-// sources of the inverse transform
-enum { C2R_SPECTRUM = 0, C2R_DELTA = 1, C2R_PSI_Z = 2 };
+// sources of the inverse transform: a stored half spectrum, the generated Gaussian field
+// or its z displacement, a k-shell of a stored spectrum times the window, the shell itself
+enum { C2R_SPECTRUM = 0, C2R_DELTA = 1, C2R_PSI_Z = 2, C2R_SHELL = 3, C2R_SHELL_ONE = 4 };
+// k-shell lo <= m < hi (m = kx^2 + ky^2 + kz^2); wt: per-axis window row of the plan
+struct ShellMask {
+  int lo, hi;
+  const float* wt;
+};
 // numpy.fft.irfftn of a half spectrum ([kz][kx][ky]; src == C2R_SPECTRUM, read from
 // spec_in, which may equal scratch) or of the generated delta / psi_z modes; scratch is a
 // half-spectrum buffer, mesh [n][n][n] float32
@@ -179,6 +185,24 @@ int populate_count(const float* field, int n, double box, double density, double
                    int64_t* total, hipStream_t s);
 int populate_place(const float* field, const float* psi, double growth, int n, double box, double bias, uint64_t seed,
                    void* ws, float* pos, int64_t capacity, hipStream_t s);
+
+// ---- bispectrum (pk_fft.hip) -----------------------------------------------------------------
+// reduction slots per shell: sum delta_i^2, sum I_i^2, sum delta1 delta2 delta_i, sum I1 I2 I_i
+constexpr int BK_SLOTS = 4;
+constexpr int BK_PART_STRIDE = 4096;   // per-slot partial capacity (>= any reduction grid)
+struct BispecArgs {
+  int nshell;
+  double box;
+  double* part;          // [nshell][BK_SLOTS][BK_PART_STRIDE]
+  double* sums;          // [nshell][BK_SLOTS] (output)
+  float* pk;             // [nshell]
+  float* bk;             // [nshell - 2]
+  float* qk;             // [nshell - 2]
+};
+// mesh -> delta_k (r2c into spec) -> per shell masked inverses (scratch) -> meshes / fused
+// reductions -> finalize; meshes m[4] = delta1, delta2, I1, I2 (n^3 float32 each)
+int fft_bispec(int n, const float* mesh, float2* spec, float2* scratch, float* const* meshes, const float2* tw,
+               const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s);
 
 // ---- configuration-space multipoles (pk_fft.hip) -------------------------------------------
 // largest bin count of the separation binning (its 64-bit accumulators live in LDS)

@@ -26,6 +26,13 @@ This is a NumPy FP64 restatement of the reference's algorithm, function by funct
   Nmodes -> inf in empty bins (xi 0 there), xi_l = (2l+1) sum / N / n^3, r = edge midpoints
   times L / n.  kedges = s_edges n / L in FP64 (the reference forms kF s n / L / kF in
   float32).
+* ``bispec``                     <- src/correlations.py:334-458: shells
+  k_min <= |k| < k_max, k_min/max = k_i/kF -/+ 1 (grid units), of k1, k2 and
+  k3(theta) = |k2 (sin t, cos t) + (0, k1)|; delta_i = irfftn(shell delta_k W),
+  I_i = irfftn(shell); P_i = sum delta_i^2 / sum I_i^2 (L/n^2)^3,
+  B = sum delta1 delta2 delta3 / sum I1 I2 I3 (L^2/n^3)^3, Q = B / (P1P2 + P1P3 + P2P3).
+  Shell membership is decided on integer m by FP64 thresholds (the reference compares
+  float32 |k| with float32 bounds).
 * ``xi_vec_fundamental``         <- src/correlations.py:191-260: slots int(|s|) = isqrt(m),
   slot 0 dropped, r = mean |s| times L / n, 0 / 0 = NaN in empty slots.
 
@@ -629,3 +636,55 @@ def xi_vec_fundamental(delta, box_size, p=2):
         r = ksum / cnt * (float(box_size) / n)
         scale = A / cnt[:, None] * f
     return r, xi, cnt.astype(F64), {"counts": cnt, "scale": scale}
+
+
+# ------------------------------------------------------------------------------------
+# bispectrum (src/correlations.py:334-458)
+# ------------------------------------------------------------------------------------
+def bispec_shells(box_size, k1, k2, theta):
+    """k_all = (k1, k2, k3(theta)) and shell bounds in grid units (:352-359)."""
+    kF = 2.0 * np.pi / float(box_size)
+    theta = np.asarray(theta, F64)
+    k3 = np.sqrt((k2 * np.sin(theta)) ** 2 + (k2 * np.cos(theta) + k1) ** 2)
+    k_all = np.concatenate([[k1, k2], k3]).astype(F64)
+    return k_all, (k_all - kF) / kF, (k_all + kF) / kF
+
+
+def bispec(delta, box_size, k1, k2, theta, p=1):
+    """FP64 restatement of src/correlations.py:334-458: (k_all, Pk[bins+2], theta, B, Q) and
+    extras (per-shell sums and the scale of the triple-product sums).  p is the window row:
+    delta_k is multiplied by prod (x / sin x)^(2p), so p = 1 is the reference's CIC
+    correction of index 2 (:362-373)."""
+    delta = np.asarray(delta, F64)
+    n = delta.shape[0]
+    k_all, kmin, kmax = bispec_shells(box_size, k1, k2, theta)
+    dk = np.fft.rfftn(delta, axes=(0, 1, 2)) * window_correction(n, 2 * p)  # :362-373
+    kx, ky, kz = half_spectrum_grids(n)
+    m = (kx * kx + ky * ky + kz * kz).astype(np.int64)
+    k = np.sqrt(m.astype(F64))
+    fields = []
+    for lo, hi in zip(kmin, kmax):
+        mask = (k >= lo) & (k < hi)                                          # :377
+        d = np.fft.irfftn(np.where(mask, dk, 0.0), s=(n, n, n), axes=(0, 1, 2))
+        I = np.fft.irfftn(mask.astype(F64), s=(n, n, n), axes=(0, 1, 2))
+        fields.append((d, I))
+    norm_p = (box_size / n ** 2) ** 3
+    norm_b = (box_size ** 2 / n ** 3) ** 3
+    Pk = np.array([(d * d).sum() / (I * I).sum() * norm_p for d, I in fields])
+    d1, I1 = fields[0]
+    d2, I2 = fields[1]
+    B, Q, trip, tri, scale = [], [], [], [], []
+    for i in range(2, len(fields)):
+        d3, I3 = fields[i]
+        t = (d1 * d2 * d3).sum()
+        c = (I1 * I2 * I3).sum()
+        b = t / c * norm_b
+        B.append(b)
+        Q.append(b / (Pk[0] * Pk[1] + Pk[0] * Pk[i] + Pk[1] * Pk[i]))
+        trip.append(t)
+        tri.append(c)
+        scale.append(np.abs(d1 * d2 * d3).sum() / abs(c) * norm_b)
+    extra = {"sum_d2": np.array([(d * d).sum() for d, _ in fields]),
+             "sum_I2": np.array([(I * I).sum() for _, I in fields]),
+             "triples": np.array(trip), "triangles": np.array(tri), "scale_B": np.array(scale)}
+    return k_all, Pk, np.asarray(theta, F64), np.array(B), np.array(Q), extra

@@ -65,6 +65,29 @@ def test_invalid_arguments_fail_before_any_launch(lib):
     assert lib.pk_plan_create(16, None) == _lib.PK_E_INVALID
 
 
+def test_extension_entry_points_validate_before_any_launch(lib):
+    nbytes = C.c_size_t()
+    # mock generator
+    assert lib.pk_gaussian_workspace_size(7, C.byref(nbytes)) == _lib.PK_E_INVALID
+    assert lib.pk_gaussian_workspace_size(64, C.byref(nbytes)) == _lib.PK_OK
+    assert nbytes.value >= (3 * 32 * 32 + 1) * 4
+    assert lib.pk_populate_workspace_size(0, C.byref(nbytes)) == _lib.PK_E_INVALID
+    assert lib.pk_populate_workspace_size(2048, C.byref(nbytes)) == _lib.PK_E_INVALID
+    assert lib.pk_populate_workspace_size(12, C.byref(nbytes)) == _lib.PK_OK
+    assert nbytes.value >= 12 ** 3 // 2 * 2
+    assert lib.pk_populate_count(None, 16, 1.0, 1.0, 0.0, 1, None, 0, None, None) == _lib.PK_E_INVALID
+    assert lib.pk_populate_count(C.c_void_p(256), 16, 1.0, -1.0, 0.0, 1, None, 0, None, None) == _lib.PK_E_INVALID
+    assert lib.pk_populate_count(C.c_void_p(256), 16, 1.0, 1.0, 0.0, 1, None, 0, None, None) == _lib.PK_E_WORKSPACE
+    assert lib.pk_populate_place(C.c_void_p(256), None, 0.0, 16, 1.0, 0.0, 1, None, 0, None, -1, None) == \
+        _lib.PK_E_INVALID
+    # plan-based entry points reject a null plan
+    assert lib.pk_gaussian_field(None, 1.0, None, None, 2, 0, 1, None, None, 0, None) == _lib.PK_E_INVALID
+    assert lib.pk_c2r(None, None, None, None, None) == _lib.PK_E_INVALID
+    assert lib.pk_gaussian_mesh(None, 1.0, None, None, 2, 0, 1, 9, None, None, None, 0, None) == _lib.PK_E_INVALID
+    assert lib.pk_xi_workspace_size(None, 4, C.byref(nbytes)) == _lib.PK_E_INVALID
+    assert lib.pk_bispec_workspace_size(None, 4, C.byref(nbytes)) == _lib.PK_E_INVALID
+
+
 def test_python_error_mapping(lib):
     with pytest.raises(ValueError):
         _lib.check("pk_scatter_workspace_size", lib.pk_scatter_workspace_size(1, 1, 0, None, None))
