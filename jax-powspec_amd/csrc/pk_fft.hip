@@ -886,17 +886,18 @@ __global__ void __launch_bounds__(PAIR_NT) k_bispec_pair(float* __restrict__ p, 
 
 // P_i = sum delta_i^2 / sum I_i^2 (L/n^2)^3, B = sum delta1 delta2 delta3 / sum I1 I2 I3
 // (L^2/n^3)^3, Q = B / (P1 P2 + P1 P3 + P2 P3) (src/correlations.py:392-452)
-__global__ void __launch_bounds__(64) k_bispec_finalize(BispecArgs a, int n, int g_pair, int g_red) {
-  const int sh = blockIdx.x;
-  if (sh >= a.nshell) return;
+constexpr int BKF_NT = 256;
+__global__ void __launch_bounds__(BKF_NT) k_bispec_finalize(BispecArgs a, int n, int g_pair, int g_red) {
+  __shared__ double red[BKF_NT / 64 + 1];
+  const int sh = blockIdx.x;   // one block per shell (uniform exit)
   double v[BK_SLOTS];
   for (int t = 0; t < BK_SLOTS; ++t) {
     const int cnt = (sh < 2) ? (t < 2 ? g_pair : 0) : g_red;
     const double* pp = a.part + ((size_t)sh * BK_SLOTS + t) * BK_PART_STRIDE;
-    double s = 0.0;
-    if (threadIdx.x == 0)
-      for (int i = 0; i < cnt; ++i) s += pp[i];
-    v[t] = s;
+    double s = 0.0;   // fixed order: strided per thread, then the block's fixed-order sum
+    for (int i = threadIdx.x; i < cnt; i += BKF_NT) s += pp[i];
+    v[t] = block_sum_f64_waves<BKF_NT>(s, red);
+    __syncthreads();
   }
   if (threadIdx.x != 0) return;
   for (int t = 0; t < BK_SLOTS; ++t) a.sums[sh * BK_SLOTS + t] = v[t];
@@ -1321,7 +1322,7 @@ template <int N> struct FFTLaunch {
         hipLaunchKernelGGL(k_zpass_c2r_reduce<N>, dim3(gr), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,
                            (const float*)meshes[one * 2], slot(sh, one), slot(sh, 2 + one));
       }
-    hipLaunchKernelGGL(k_bispec_finalize, dim3(a.nshell), dim3(64), 0, s, a, N, gp, gr);
+    hipLaunchKernelGGL(k_bispec_finalize, dim3(a.nshell), dim3(BKF_NT), 0, s, a, N, gp, gr);
     if (a.nshell > 2)
       hipLaunchKernelGGL(k_bispec_bq, dim3((a.nshell - 2 + 63) / 64), dim3(64), 0, s, a, N);
     return check_hip(hipGetLastError(), "bispectrum launch");
