@@ -248,14 +248,17 @@ int pk_xi_mesh(pk_plan* plan, const float* mesh, const int32_t* thr, const int32
  *     m = kx^2 + ky^2 + kz^2 (jax_powspec_amd.correlations derives them from
  *     k_i / kF -/+ 1 in FP64); 2 <= nshell <= PK_BISPEC_MAX_SHELLS;
  *   - window_p: the reference multiplies delta_k by prod (x / sin x)^2, i.e. p = 1;
+ *   - i_sums (nullable, device FP64 [nshell][2]): sum I_i^2 and sum I1 I2 I_i (0 for i < 2),
+ *     which depend only on n and the shells, e.g. columns 1 and 3 of an earlier call's sums;
+ *     when given, the indicator fields are not transformed (half the inverse FFTs);
  *   - outputs: sums FP64 [nshell][4] (sum delta^2, sum I^2, triple sums), pk float32
  *     [nshell], bk and qk float32 [nshell - 2]; workspace: pk_bispec_workspace_size.
  */
 #define PK_BISPEC_MAX_SHELLS 1024
 int pk_bispec_workspace_size(pk_plan* plan, int nshell, size_t* bytes);
 int pk_bispec_mesh(pk_plan* plan, const float* mesh, const int32_t* shell_lo, const int32_t* shell_hi, int nshell,
-                   int window_p, double box_size, double* sums, float* pk, float* bk, float* qk, void* workspace,
-                   size_t workspace_bytes, void* stream);
+                   int window_p, double box_size, const double* i_sums, double* sums, float* pk, float* bk,
+                   float* qk, void* workspace, size_t workspace_bytes, void* stream);
 
 /* --- seeded lognormal mocks -----------------------------------------------------------
  * The reference's mock pipeline (tests/create_lognormal.py:42-55): a Gaussian random

@@ -201,11 +201,17 @@ def bispec_shells(n, box_size, k1, k2, theta):
     return k_all, theta, lo.astype(np.int32), hi.astype(np.int32)
 
 
-def bispec(delta, box_size, k1, k2, theta, *, mas="CIC", full=False):
+# sum I_i^2 and sum I1 I2 I_i of the bispectrum's shell indicators per (device, n, shells):
+# they do not depend on the field, so later calls skip the indicator transforms
+_BISPEC_I_SUMS: dict = {}
+
+
+def bispec(delta, box_size, k1, k2, theta, *, mas="CIC", full=False, reuse_shell_sums=True):
     """Bispectrum and reduced bispectrum of (k1, k2, theta) triangles (src/correlations.py:
     334-458).  Returns (k_all[bins+2], Pk[bins+2], theta, B[bins], Q[bins]) as float32
     device tensors (k_all and theta from the host computation).  mas: 'CIC' (the
-    reference's (x / sin x)^2 per axis on delta_k) or 'NONE'."""
+    reference's (x / sin x)^2 per axis on delta_k) or 'NONE'.  The shell indicators' sums
+    are kept per (device, n, shells) after the first call and reused (reuse_shell_sums)."""
     key = "NONE" if mas is None else str(mas).upper()
     if key not in ("CIC", "CIC_REF", "NONE"):
         raise ValueError(f"bispec supports mas='CIC' or 'NONE', got {mas!r}")
@@ -220,6 +226,8 @@ def bispec(delta, box_size, k1, k2, theta, *, mas="CIC", full=False):
         raise ValueError(f"at most {_lib.BISPEC_MAX_SHELLS - 2} theta bins")
     lib = _lib.load()
     nbytes = C.c_size_t()
+    key = (str(device), n, lo.tobytes(), hi.tobytes())
+    isums = _BISPEC_I_SUMS.get(key) if reuse_shell_sums else None
     with torch.cuda.device(device):
         _lib.check("pk_bispec_workspace_size", lib.pk_bispec_workspace_size(plan.handle, ns, C.byref(nbytes)))
         ws = _device.workspace(nbytes.value, device)
@@ -231,8 +239,12 @@ def bispec(delta, box_size, k1, k2, theta, *, mas="CIC", full=False):
         hi_h = (C.c_int32 * ns)(*hi.tolist())
         _lib.check("pk_bispec_mesh", lib.pk_bispec_mesh(
             plan.handle, _device.ptr(mesh), C.cast(lo_h, C.c_void_p), C.cast(hi_h, C.c_void_p), ns, p,
-            float(box_size), _device.ptr(sums), _device.ptr(pk), _device.ptr(bk), _device.ptr(qk), _device.ptr(ws),
-            nbytes.value, _device.stream_ptr(device)))
+            float(box_size), _device.ptr(isums), _device.ptr(sums), _device.ptr(pk), _device.ptr(bk), _device.ptr(qk),
+            _device.ptr(ws), nbytes.value, _device.stream_ptr(device)))
+        if reuse_shell_sums and isums is None:
+            if len(_BISPEC_I_SUMS) > 256:
+                _BISPEC_I_SUMS.clear()
+            _BISPEC_I_SUMS[key] = sums[:, [1, 3]].contiguous()
     f32 = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=device)
     res = (f32(k_all), pk, f32(th), bk[:ns - 2], qk[:ns - 2])
     return res + ({"sums": sums, "shell_lo": lo, "shell_hi": hi},) if full else res

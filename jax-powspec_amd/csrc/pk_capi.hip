@@ -1070,8 +1070,8 @@ int pk_bispec_workspace_size(pk_plan* p, int nshell, size_t* bytes) {
 }
 
 int pk_bispec_mesh(pk_plan* p, const float* mesh, const int32_t* shell_lo, const int32_t* shell_hi, int nshell,
-                   int window_p, double box_size, double* sums, float* pk, float* bk, float* qk, void* workspace,
-                   size_t workspace_bytes, void* stream) {
+                   int window_p, double box_size, const double* i_sums, double* sums, float* pk, float* bk, float* qk,
+                   void* workspace, size_t workspace_bytes, void* stream) {
   int e = check_plan(p);
   if (e) return e;
   const int n = p->n;
@@ -1096,6 +1096,7 @@ int pk_bispec_mesh(pk_plan* p, const float* mesh, const int32_t* shell_lo, const
   a.nshell = nshell;
   a.box = box_size;
   a.part = reinterpret_cast<double*>(w + 2 * half + 4 * msz);
+  a.i_sums = i_sums;
   a.sums = sums;
   a.pk = pk;
   a.bk = bk;

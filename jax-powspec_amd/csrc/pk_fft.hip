@@ -900,6 +900,10 @@ __global__ void __launch_bounds__(BKF_NT) k_bispec_finalize(BispecArgs a, int n,
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
+  if (a.i_sums) {   // indicator sums depend only on n and the shells: taken as given
+    v[1] = a.i_sums[2 * sh];
+    v[3] = sh >= 2 ? a.i_sums[2 * sh + 1] : 0.0;
+  }
   for (int t = 0; t < BK_SLOTS; ++t) a.sums[sh * BK_SLOTS + t] = v[t];
   const double h = a.box / ((double)n * (double)n);
   const double norm_p = h * h * h;
@@ -1299,9 +1303,10 @@ template <int N> struct FFTLaunch {
     if (!e) e = ypass(spec, tw, s);
     if (!e) e = xpass(spec, tw, s);
     if (e) return e;
-    // first two shells: stored fields
+    // first two shells: stored fields (the indicator fields only when their sums are unknown)
+    const int nones = a.i_sums ? 1 : 2;
     for (int sh = 0; sh < 2; ++sh)
-      for (int one = 0; one < 2; ++one) {
+      for (int one = 0; one < nones; ++one) {
         e = shell_inverse(one != 0, spec, scratch, tw, ShellMask{lo[sh], hi[sh], wt}, s);
         if (e) return e;
         hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,
@@ -1312,11 +1317,12 @@ template <int N> struct FFTLaunch {
     auto slot = [&](int sh, int t) { return a.part + ((size_t)sh * BK_SLOTS + t) * BK_PART_STRIDE; };
     hipLaunchKernelGGL(k_bispec_pair, dim3(gp), dim3(PAIR_NT), 0, s, meshes[0], (const float*)meshes[1], total,
                        slot(0, 0), slot(1, 0));
-    hipLaunchKernelGGL(k_bispec_pair, dim3(gp), dim3(PAIR_NT), 0, s, meshes[2], (const float*)meshes[3], total,
-                       slot(0, 1), slot(1, 1));
+    if (!a.i_sums)
+      hipLaunchKernelGGL(k_bispec_pair, dim3(gp), dim3(PAIR_NT), 0, s, meshes[2], (const float*)meshes[3], total,
+                         slot(0, 1), slot(1, 1));
     const int gr = red_grid();
     for (int sh = 2; sh < a.nshell; ++sh)
-      for (int one = 0; one < 2; ++one) {
+      for (int one = 0; one < nones; ++one) {
         e = shell_inverse(one != 0, spec, scratch, tw, ShellMask{lo[sh], hi[sh], wt}, s);
         if (e) return e;
         hipLaunchKernelGGL(k_zpass_c2r_reduce<N>, dim3(gr), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,

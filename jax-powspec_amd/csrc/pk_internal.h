@@ -194,6 +194,7 @@ struct BispecArgs {
   int nshell;
   double box;
   double* part;          // [nshell][BK_SLOTS][BK_PART_STRIDE]
+  const double* i_sums;  // nullable [nshell][2]: known sum I_i^2, sum I1 I2 I_i (no I transforms)
   double* sums;          // [nshell][BK_SLOTS] (output)
   float* pk;             // [nshell]
   float* bk;             // [nshell - 2]

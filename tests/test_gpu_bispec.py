@@ -24,7 +24,8 @@ def field(n, seed):
 
 @pytest.mark.parametrize("n,L,f1,f2,nth,mas", [(16, 200.0, 2.0, 3.0, 5, "CIC"), (32, 500.0, 3.0, 5.0, 9, "CIC"),
                                                (48, 700.0, 4.0, 6.5, 6, "NONE"), (64, 1000.0, 5.0, 8.0, 15, "CIC"),
-                                               (128, 1000.0, 10.0, 20.0, 8, "CIC")])
+                                               (128, 1000.0, 10.0, 20.0, 8, "CIC"),
+                                               (96, 900.0, 7.0, 12.0, 7, "CIC"), (24, 300.0, 2.5, 4.0, 4, "CIC")])
 def test_bispec_parity(gpu, n, L, f1, f2, nth, mas):
     print(f"bispec n={n} shells={nth + 2}", flush=True)
     d = field(n, n)
@@ -41,6 +42,21 @@ def test_bispec_parity(gpu, n, L, f1, f2, nth, mas):
     den = o_b / o_q
     tol_q = tol_b / np.abs(den) + 1e-5 * np.abs(o_q)
     assert np.all(np.abs(qk - o_q) <= tol_q), np.max(np.abs(qk - o_q) / tol_q)
+
+
+def test_bispec_shell_sums_reuse_is_bit_identical(gpu):
+    n, L = 64, 800.0
+    d = torch.as_tensor(field(n, 7), device=gpu)
+    kF = 2 * np.pi / L
+    theta = np.linspace(0, np.pi, 6)
+    fresh = correlations.bispec(d, L, 4 * kF, 7 * kF, theta, reuse_shell_sums=False, full=True)
+    first = correlations.bispec(d, L, 4 * kF, 7 * kF, theta)      # fills or hits the cache
+    again = correlations.bispec(d, L, 4 * kF, 7 * kF, theta, full=True)
+    for x, y in zip(fresh[:5], again[:5]):
+        assert torch.equal(x, y)
+    assert torch.equal(fresh[5]["sums"], again[5]["sums"])
+    for x, y in zip(first, again[:5]):
+        assert torch.equal(x, y)
 
 
 def test_bispec_full_size_runs_and_is_deterministic(gpu):

@@ -73,7 +73,8 @@ def test_irfftn_parity(gpu, n):
     assert torch.equal(t, before)     # the input is only read
 
 
-@pytest.mark.parametrize("n,rayleigh,seed", [(16, 1, 5), (64, 0, 9), (128, 1, 2**33 + 1)])
+@pytest.mark.parametrize("n,rayleigh,seed", [(16, 1, 5), (24, 1, 6), (64, 0, 9), (96, 1, 10), (128, 1, 2**33 + 1),
+                                             (192, 0, 12)])
 def test_gaussian_mesh_fused(gpu, n, rayleigh, seed):
     say(f"gaussian_mesh n={n}")
     box = 1000.0
@@ -90,7 +91,7 @@ def test_gaussian_mesh_fused(gpu, n, rayleigh, seed):
     assert np.abs(psi - op).max() <= 1e-5 * np.abs(op).max()
 
 
-@pytest.mark.parametrize("n", [256, 512])
+@pytest.mark.parametrize("n", [256, 384, 512])
 def test_gaussian_mesh_round_trip_full_size(gpu, n):
     """rfftn(gaussian_mesh) = gaussian_field at full size (the field is exactly Hermitian)."""
     say(f"round trip n={n}")
@@ -124,7 +125,7 @@ def _field(rng, n, kind):
 @pytest.mark.parametrize("n,kind,density,bias,seed", [
     (8, "lognormal", 0.5, 0.0, 1), (12, "lognormal", 0.2, 0.0, 2), (16, "sparse", 0.3, 0.0, 3),
     (24, "gauss", 0.05, 1.3, 4), (32, "lognormal", 0.002, 0.0, 2**36 + 7), (48, "sparse", 0.01, 0.0, 8),
-    (64, "gauss", 0.004, 1.1, 100)])
+    (64, "gauss", 0.004, 1.1, 100), (96, "lognormal", 0.0005, 0.0, 17), (7, "lognormal", 1.0, 0.0, 21)])
 def test_populate_parity(gpu, n, kind, density, bias, seed):
     box = 250.0 + 10 * n
     say(f"populate n={n} kind={kind} density={density} bias={bias}")

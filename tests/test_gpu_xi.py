@@ -53,7 +53,7 @@ def test_xi_vec_parity(gpu, n, box, edges):
     check(out, R.xi_vec(d, box, edges), n, d)
 
 
-@pytest.mark.parametrize("n", [8, 16, 64, 128, 256])
+@pytest.mark.parametrize("n", [8, 12, 16, 48, 64, 128, 192, 256])
 def test_xi_fundamental_parity(gpu, n):
     print(f"xi_vec_fundamental n={n}", flush=True)
     box = 700.0
