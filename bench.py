@@ -574,11 +574,14 @@ def bench_bispec(dev, reps=3, cpu=True):
     ms = timed(True)           # indicator sums reused (every later call with these shells)
     half = 8 * n * n * (n // 2 + 1)
     mesh = 4 * n ** 3
-    nsh = theta.size + 2
-    # forward transform; per shell two inverse x + y passes and a C2R (two stored for the first
-    # two shells, product-mesh reads for the rest); the pair products
-    # (with reused indicator sums only the delta half of the shell work remains)
-    byts = (mesh + 5 * half) + nsh * (4 * half) + 2 * (half + mesh) + (nsh - 2) * (half + mesh) + 3 * mesh
+    _, _, _, hi = correlations.bispec_shells(n, box, k1, k2, theta)
+    nsh = hi.size
+    # forward transform; then, with reused indicator sums, one delta field per shell. A
+    # shell's inverse passes touch only the planes kz <= isqrt(hi - 1): the x pass writes
+    # them, the inverse y pass reads and writes them, the C2R reads them and writes a mesh
+    # (first two shells) or reads the product mesh (the others); the pair products
+    planes = [min(math.isqrt(int(h) - 1), n // 2) + 1 if h > 0 else 0 for h in hi]
+    byts = (mesh + 5 * half) + sum(4 * 8 * n * n * p for p in planes) + 2 * mesh + (nsh - 2) * mesh + 3 * mesh
     gbs = byts / (ms * 1e-3) / 1e9
     out = {"workload": f"bispec {n}^3, L={box:g}, k1={k1}, k2={k2}, {theta.size} angles ({nsh} shells)",
            "ms": round(ms, 2), "per_s": round(1e3 / ms, 2), "ms_first_call": round(ms_all, 2),

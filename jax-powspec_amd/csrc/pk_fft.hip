@@ -109,14 +109,16 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
 // y pass: contiguous rows of N complex, in place
 // ------------------------------------------------------------------------------------
 // CONJ: conj on load and store, i.e. the unnormalised inverse transform (e^{+i})
+// ntiles < 0: every tile; otherwise the first ntiles (rows [kz][x] of the lowest planes)
 template <int N, bool CONJ = false>
-__device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const float2* __restrict__ twg, int bx, int nbx) {
+__device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const float2* __restrict__ twg, int bx, int nbx,
+                                           int ntiles = -1) {
   using Cf = FFTCfg<N>;
   // rows are skewed in LDS (point p at p + p / 8), which keeps the butterflies' strided
   // writes off each other's banks; twiddles from a quarter table
   constexpr int SK = 3;
   constexpr int C = Cf::CY, NT = Cf::NTY, CS = N + (SK > 0 ? (N >> SK) : 0) + 1;
-  constexpr int NTILES = (N / 2 + 1) * N / C;
+  const int NTILES = ntiles < 0 ? (N / 2 + 1) * N / C : ntiles;
   __shared__ float2 buf[C * CS];
   __shared__ float2 twq[N / 4];
   const TwQuarter<N> tw{twq};
@@ -626,10 +628,11 @@ k_xpass_adjoint(Job j, const float2* __restrict__ twg, const float* __restrict__
   }
 }
 
+// kz_lim: planes above it are not transformed (they are zero for the caller)
 template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_inverse(float2* __restrict__ spec,
-                                                                  const float2* __restrict__ twg) {
-  ypass_body<N, true>(spec, twg, blockIdx.x, gridDim.x);
+                                                                  const float2* __restrict__ twg, int kz_lim) {
+  ypass_body<N, true>(spec, twg, blockIdx.x, gridDim.x, (kz_lim + 1) * N / FFTCfg<N>::CY);
 }
 
 // Half-spectrum rows B[kz][x][y] (kz = 0..N/2, transformed in x and y) -> real rows
@@ -637,9 +640,10 @@ __global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_inverse(float2* __rest
 // transform of Y_0 = Re B_0, Y_{N/2} = Re B_{N/2}, Y_k = B_k / 2: an N/2-point complex
 // inverse FFT of Z_k = (Y_k + conj Y_{N/2-k}) + i e^{2 pi i k / N} (Y_k - conj Y_{N/2-k}),
 // whose outputs are u(2j) + i u(2j+1).
+// planes kz > kz_lim are taken as zero (not read)
 template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTZ)
-k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, float* __restrict__ out) {
+k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, float* __restrict__ out, int kz_lim) {
   using Cf = FFTCfg<N>;
   constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
   constexpr int YT = N / C;
@@ -656,7 +660,9 @@ k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, fl
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int i = tid + k * NT;
-        if (TOT % NT == 0 || i < TOT) v[k] = bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)];
+        if (TOT % NT == 0 || i < TOT)
+          v[k] = (i / C) <= kz_lim ? bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)]
+                                   : make_float2(0.0f, 0.0f);
       }
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
@@ -714,7 +720,7 @@ k_zpass_c2r(const float2* __restrict__ bspec, const float2* __restrict__ twg, fl
 template <int N, int SRC>
 __global__ void __launch_bounds__(FFTCfg<N>::NTX)
 k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, const float2* __restrict__ twg,
-                FieldGen g, ShellMask sm) {
+                FieldGen g, ShellMask sm, int kz_lim) {
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
   constexpr int YT = N / C;
@@ -726,7 +732,8 @@ k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, co
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   const float s_edge = (float)(1.0 / ((double)N * (double)N * (double)N));
   const float s_int = 2.0f * s_edge;
-  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+  const int ntiles = min(NTILES, (kz_lim + 1) * YT);   // planes above kz_lim: zero, not written
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int kz = tile / YT, yt = tile % YT;
     const float sc = (kz == 0 || kz == H) ? s_edge : s_int;
     if constexpr (SRC == C2R_SPECTRUM) {
@@ -792,7 +799,7 @@ k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, co
 template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTZ)
 k_zpass_c2r_reduce(const float2* __restrict__ bspec, const float2* __restrict__ twg, const float* __restrict__ prod,
-                   double* __restrict__ part_sq, double* __restrict__ part_prod) {
+                   double* __restrict__ part_sq, double* __restrict__ part_prod, int kz_lim) {
   using Cf = FFTCfg<N>;
   constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
   constexpr int YT = N / C;
@@ -811,7 +818,9 @@ k_zpass_c2r_reduce(const float2* __restrict__ bspec, const float2* __restrict__ 
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int i = tid + k * NT;
-        if (TOT % NT == 0 || i < TOT) v[k] = bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)];
+        if (TOT % NT == 0 || i < TOT)
+          v[k] = (i / C) <= kz_lim ? bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)]
+                                   : make_float2(0.0f, 0.0f);
       }
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
@@ -1257,16 +1266,16 @@ template <int N> struct FFTLaunch {
     if (!gz) gz = persistent_grid(k_zpass_c2r<N>, Cf::NTZ, ZTILES);
     StageTimer t(ST_ADJOINT_FFT, s);
     hipLaunchKernelGGL(k_xpass_adjoint<N>, dim3(gx), dim3(Cf::NTX), 0, s, j, tw, coef, bspec);
-    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, bspec, tw);
-    hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)bspec, tw, gmesh);
+    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, bspec, tw, N / 2);
+    hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)bspec, tw, gmesh, N / 2);
     return check_hip(hipGetLastError(), "adjoint FFT launch");
   }
   template <int SRC>
   static int xinv(const float2* in, float2* scratch, const float2* tw, const FieldGen& g, hipStream_t s,
-                  ShellMask sm = ShellMask{0, 0, nullptr}) {
+                  ShellMask sm = ShellMask{0, 0, nullptr}, int kz_lim = N / 2) {
     static int gx = 0;
     if (!gx) gx = persistent_grid(k_xpass_inverse<N, SRC>, Cf::NTX, XTILES);
-    hipLaunchKernelGGL((k_xpass_inverse<N, SRC>), dim3(gx), dim3(Cf::NTX), 0, s, in, scratch, tw, g, sm);
+    hipLaunchKernelGGL((k_xpass_inverse<N, SRC>), dim3(gx), dim3(Cf::NTX), 0, s, in, scratch, tw, g, sm, kz_lim);
     return check_hip(hipGetLastError(), "inverse x pass launch");
   }
   static int red_grid() {
@@ -1284,15 +1293,23 @@ template <int N> struct FFTLaunch {
     return g;
   }
   // shell inverse: x pass (masked source) + inverse y, into scratch
+  // last plane a shell lo <= m < hi can touch: kz^2 <= m < hi
+  static int shell_kz_lim(int hi) {
+    if (hi <= 0) return -1;
+    int k = 0;
+    while ((k + 1) * (k + 1) <= hi - 1) ++k;
+    return k < N / 2 ? k : N / 2;
+  }
   static int shell_inverse(bool one, const float2* spec, float2* scratch, const float2* tw, ShellMask sm,
                            hipStream_t s) {
     static int gy = 0;
     if (!gy) gy = persistent_grid(k_ypass_inverse<N>, Cf::NTY, YTILES);
     FieldGen g{};
     g.n = N;
-    int e = one ? xinv<C2R_SHELL_ONE>(spec, scratch, tw, g, s, sm) : xinv<C2R_SHELL>(spec, scratch, tw, g, s, sm);
+    const int kl = shell_kz_lim(sm.hi);
+    int e = one ? xinv<C2R_SHELL_ONE>(spec, scratch, tw, g, s, sm, kl) : xinv<C2R_SHELL>(spec, scratch, tw, g, s, sm, kl);
     if (e) return e;
-    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw);
+    if (kl >= 0) hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw, kl);
     return check_hip(hipGetLastError(), "shell inverse launch");
   }
   static int bispec(const float* mesh, float2* spec, float2* scratch, float* const* meshes, const float2* tw,
@@ -1310,7 +1327,7 @@ template <int N> struct FFTLaunch {
         e = shell_inverse(one != 0, spec, scratch, tw, ShellMask{lo[sh], hi[sh], wt}, s);
         if (e) return e;
         hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,
-                           meshes[one * 2 + sh]);
+                           meshes[one * 2 + sh], shell_kz_lim(hi[sh]));
       }
     const int gp = pair_grid();
     const int64_t total = (int64_t)N * N * N;
@@ -1326,7 +1343,7 @@ template <int N> struct FFTLaunch {
         e = shell_inverse(one != 0, spec, scratch, tw, ShellMask{lo[sh], hi[sh], wt}, s);
         if (e) return e;
         hipLaunchKernelGGL(k_zpass_c2r_reduce<N>, dim3(gr), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,
-                           (const float*)meshes[one * 2], slot(sh, one), slot(sh, 2 + one));
+                           (const float*)meshes[one * 2], slot(sh, one), slot(sh, 2 + one), shell_kz_lim(hi[sh]));
       }
     hipLaunchKernelGGL(k_bispec_finalize, dim3(a.nshell), dim3(BKF_NT), 0, s, a, N, gp, gr);
     if (a.nshell > 2)
@@ -1342,8 +1359,8 @@ template <int N> struct FFTLaunch {
             : src == C2R_DELTA  ? xinv<C2R_DELTA>(in, scratch, tw, g, s)
                                 : xinv<C2R_PSI_Z>(in, scratch, tw, g, s);
     if (e) return e;
-    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw);
-    hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw, mesh);
+    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw, N / 2);
+    hipLaunchKernelGGL(k_zpass_c2r<N>, dim3(gz), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw, mesh, N / 2);
     return check_hip(hipGetLastError(), "inverse FFT launch");
   }
   static int xi_gx() {
@@ -1361,7 +1378,7 @@ template <int N> struct FFTLaunch {
     if (e) return e;
     hipLaunchKernelGGL(k_xpass_power_inverse<N>, dim3(xi_gx()), dim3(Cf::NTX), 0, s, spec, tw, wt, a.xi0_part);
     hipLaunchKernelGGL(k_xi_scale, dim3(1), dim3(256), 0, s, a, N);
-    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, spec, tw);
+    hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, spec, tw, N / 2);
     hipLaunchKernelGGL(k_zpass_c2r_bin<N>, dim3(gz), dim3(Cf::NTZ), (size_t)a.nb * 3 * sizeof(unsigned long long), s,
                        (const float2*)spec, tw, a);
     hipLaunchKernelGGL(k_xi_finalize, dim3((a.nb + 255) / 256), dim3(256), 0, s, a, N);
