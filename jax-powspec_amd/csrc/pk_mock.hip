@@ -285,11 +285,34 @@ __global__ void __launch_bounds__(POP_NT) k_pop_mark(PopK a) {
     }
     long long rtot;
     // two scan buffers: round k + 1 writes the other one, so one barrier per round
-    long long p = base + block_scan_i64<POP_NT>((long long)n0 + n1, sh[k & 1], rtot);
+    const long long p = base + block_scan_i64<POP_NT>((long long)n0 + n1, sh[k & 1], rtot);
     base += rtot;
-    for (int t = 0; t < n0 + n1; ++t, ++p) {
-      if (p >= a.capacity) break;
-      rec[3 * p] = (uint32_t)(t < n0 ? c : c + 1);
+    // the wave's particles occupy the contiguous slots [p(lane 0), p(lane 63) + count(63));
+    // they are written lane by lane (coalesced, no divergence): the owner of a slot is the
+    // last lane whose first slot is not above it (lanes without particles share the next
+    // lane's offset, and an owner's successors start past its slots)
+    const int lane = threadIdx.x & 63;
+    const long long w0 = __shfl(p, 0, 64);
+    const int off = (int)(p - w0);
+    const int T = (int)(__shfl(p + n0 + n1, 63, 64) - w0);
+    const int64_t pair0 = c - 2 * (int64_t)lane;        // the wave's first cell
+    for (int s0 = 0; s0 < T; s0 += 64) {                // wave-uniform trip count
+      const int sidx = s0 + lane;
+      int lo = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {        // largest lane L with off(L) <= sidx
+        const int probe = __shfl(off, lo + step, 64);
+        if (probe <= sidx) lo += step;
+      }
+      const int o = __shfl(off, lo, 64);
+      const int first = __shfl(n0, lo, 64);
+      if (sidx < T) {
+        const long long slot = w0 + sidx;
+        if (slot < a.capacity) {
+          const int64_t cell = pair0 + 2 * (int64_t)lo + (sidx - o < first ? 0 : 1);
+          rec[3 * slot] = (uint32_t)cell;
+        }
+      }
     }
   }
 }
