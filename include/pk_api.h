@@ -244,6 +244,8 @@ int pk_xi_mesh(pk_plan* plan, const float* mesh, const int32_t* thr, const int32
  * (k1, k2, then k3(theta)), delta_i = irfftn(shell delta_k W) and I_i = irfftn(shell);
  * P_i = sum delta_i^2 / sum I_i^2 (L/n^2)^3, and for i >= 2
  * B = sum delta1 delta2 delta_i / sum I1 I2 I_i (L^2/n^3)^3, Q = B / (P1P2 + P1Pi + P2Pi).
+ * The real-space sums of shells i >= 2 are evaluated in Fourier space (Parseval, against
+ * the spectra of delta1 delta2 and I1 I2), which is exact up to rounding.
  *   - shell_lo / shell_hi: HOST int32[nshell], shell i = {lo_i <= m < hi_i} on
  *     m = kx^2 + ky^2 + kz^2 (jax_powspec_amd.correlations derives them from
  *     k_i / kF -/+ 1 in FP64); 2 <= nshell <= PK_BISPEC_MAX_SHELLS;

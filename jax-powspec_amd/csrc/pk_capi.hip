@@ -1053,7 +1053,7 @@ namespace {
 size_t bispec_ws_bytes(int n, int nshell) {
   const size_t half = align_up((size_t)(n / 2 + 1) * n * n * sizeof(float2));
   const size_t mesh = align_up((size_t)n * n * n * sizeof(float));
-  return 2 * half + 4 * mesh + align_up((size_t)nshell * BK_SLOTS * BK_PART_STRIDE * sizeof(double));
+  return 3 * half + 4 * mesh + align_up((size_t)nshell * BK_SLOTS * BK_PART_STRIDE * sizeof(double));
 }
 }  // namespace
 
@@ -1090,19 +1090,20 @@ int pk_bispec_mesh(pk_plan* p, const float* mesh, const int32_t* shell_lo, const
   const size_t msz = align_up((size_t)n * n * n * sizeof(float));
   float2* spec = reinterpret_cast<float2*>(w);
   float2* scratch = reinterpret_cast<float2*>(w + half);
+  float2* qspec = reinterpret_cast<float2*>(w + 2 * half);
   float* meshes[4];
-  for (int i = 0; i < 4; ++i) meshes[i] = reinterpret_cast<float*>(w + 2 * half + i * msz);
+  for (int i = 0; i < 4; ++i) meshes[i] = reinterpret_cast<float*>(w + 3 * half + i * msz);
   BispecArgs a{};
   a.nshell = nshell;
   a.box = box_size;
-  a.part = reinterpret_cast<double*>(w + 2 * half + 4 * msz);
+  a.part = reinterpret_cast<double*>(w + 3 * half + 4 * msz);
   a.i_sums = i_sums;
   a.sums = sums;
   a.pk = pk;
   a.bk = bk;
   a.qk = qk;
-  return fft_bispec(n, mesh, spec, scratch, meshes, p->tw, p->wtab + (size_t)window_p * n, lo.data(), hi.data(), a,
-                    as_stream(stream));
+  return fft_bispec(n, mesh, spec, scratch, qspec, meshes, p->tw, p->wtab + (size_t)window_p * n, lo.data(), hi.data(),
+                    a, as_stream(stream));
 }
 
 }  // extern "C"

@@ -791,87 +791,11 @@ k_xpass_inverse(const float2* __restrict__ spec_in, float2* __restrict__ out, co
 }
 
 // ------------------------------------------------------------------------------------
-// bispectrum (src/correlations.py:334-458): per k-shell the inverse transforms of the
-// shell's delta_k W and of its indicator; the first two shells are stored and multiplied
-// (delta1 delta2, I1 I2), later shells are reduced as their C2R rows are produced:
-// sum v^2 and sum v prod (FP64 per thread, fixed-order block sums), never stored.
+// bispectrum (src/correlations.py:334-458): the first two shells' fields (inverse
+// transforms of the shell's delta_k W and of its indicator) are stored and multiplied
+// (delta1 delta2, I1 I2); the later shells' sums are taken in Fourier space against the
+// spectra of those products (k_bispec_kspace), so they need no inverse transform.
 // ------------------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(FFTCfg<N>::NTZ)
-k_zpass_c2r_reduce(const float2* __restrict__ bspec, const float2* __restrict__ twg, const float* __restrict__ prod,
-                   double* __restrict__ part_sq, double* __restrict__ part_prod, int kz_lim) {
-  using Cf = FFTCfg<N>;
-  constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
-  constexpr int YT = N / C;
-  constexpr int NTILES = N * YT;
-  __shared__ float2 buf[C * CS];
-  __shared__ float2 tw[N];
-  __shared__ double red[NT];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
-  double ssq = 0.0, spr = 0.0;
-  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
-    const int x = tile / YT, yt = tile % YT;
-    {
-      constexpr int TOT = (NC + 1) * C, PER = (TOT + NT - 1) / NT;
-      float2 v[PER];
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * NT;
-        if (TOT % NT == 0 || i < TOT)
-          v[k] = (i / C) <= kz_lim ? bspec[((size_t)(i / C) * N + x) * N + (size_t)yt * C + (i % C)]
-                                   : make_float2(0.0f, 0.0f);
-      }
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * NT;
-        if (TOT % NT == 0 || i < TOT) buf[(i % C) * CS + i / C] = v[k];
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < (NC / 2 + 1) * C; i += NT) {
-      const int c = i % C, k = i / C;
-      const int kk = NC - k;
-      float2* row = buf + c * CS;
-      auto Y = [&](int q) {
-        const float2 b = row[q];
-        if (q == 0 || q == NC) return make_float2(b.x, 0.0f);
-        return make_float2(0.5f * b.x, 0.5f * b.y);
-      };
-      const float2 yk = Y(k), ykk = Y(kk);
-      auto zval = [&](const float2 p, const float2 q2, int q) {
-        const float2 sm = make_float2(p.x + q2.x, p.y - q2.y);
-        const float2 d = make_float2(p.x - q2.x, p.y + q2.y);
-        const float2 e = make_float2(tw[q].x, -tw[q].y);
-        const float2 ed = cmul(e, d);
-        return make_float2(sm.x - ed.y, sm.y + ed.x);
-      };
-      const float2 zk = zval(yk, ykk, k);
-      const float2 zkk = zval(ykk, yk, kk);
-      row[k] = make_float2(zk.x, -zk.y);
-      if (kk != k && kk < NC) row[kk] = make_float2(zkk.x, -zkk.y);
-    }
-    __syncthreads();
-    lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
-    const float2* prow = reinterpret_cast<const float2*>(prod + ((size_t)x * N + (size_t)yt * C) * N);
-    for (int i = tid; i < C * NC; i += NT) {
-      const int c = i / NC, jj = i % NC;
-      const float2 v = buf[c * CS + jj];
-      const float u0 = v.x, u1 = -v.y;              // (u(2j), u(2j+1))
-      const float2 p = prow[(size_t)c * NC + jj];
-      ssq += (double)u0 * (double)u0 + (double)u1 * (double)u1;
-      spr += (double)u0 * (double)p.x + (double)u1 * (double)p.y;
-    }
-    __syncthreads();
-  }
-  const double a = block_sum_f64(ssq, red, NT);
-  const double b = block_sum_f64(spr, red, NT);
-  if (tid == 0) {
-    part_sq[blockIdx.x] = a;
-    part_prod[blockIdx.x] = b;
-  }
-}
-
 // products of the first two shells' fields in place (p <- p q) and sums of their squares
 constexpr int PAIR_NT = 256;
 __global__ void __launch_bounds__(PAIR_NT) k_bispec_pair(float* __restrict__ p, const float* __restrict__ q,
@@ -895,6 +819,75 @@ __global__ void __launch_bounds__(PAIR_NT) k_bispec_pair(float* __restrict__ p, 
 
 // P_i = sum delta_i^2 / sum I_i^2 (L/n^2)^3, B = sum delta1 delta2 delta3 / sum I1 I2 I3
 // (L^2/n^3)^3, Q = B / (P1 P2 + P1 P3 + P2 P3) (src/correlations.py:392-452)
+// Shell sums of the later shells in Fourier space (Parseval, half spectrum with weight 1 on
+// the kz = 0, n/2 planes and 2 elsewhere): with D = delta_k W on shell i, P = rfftn(delta1
+// delta2), Q = rfftn(I1 I2),
+//   sum_x delta_i^2       = n^-3 sum |D|^2,        sum_x I_i^2     = n^-3 sum 1,
+//   sum_x delta1 delta2 delta_i = n^-3 sum Re(D conj P),  sum_x I1 I2 I_i = n^-3 sum Re Q,
+// so no shell needs its own inverse transform.  Shells s0 .. s0 + ns - 1 (ns <= BKK_CHUNK)
+// per launch, register accumulators, fixed-order block sums into the partial slots.
+constexpr int BKK_NT = 256;
+constexpr int BKK_CHUNK = 8;
+__global__ void __launch_bounds__(BKK_NT)
+k_bispec_kspace(const float2* __restrict__ dspec, const float2* __restrict__ pspec, const float2* __restrict__ qspec,
+                const float* __restrict__ wt, ShellChunk sh, int s0, int ns, int n, int kz_lim, BispecArgs a) {
+  static_assert(BKK_CHUNK == 8, "ShellChunk holds 8 shells");
+  __shared__ double red[BKK_NT / 64 + 1];
+  int slo[BKK_CHUNK], shi[BKK_CHUNK];
+  int mlo = 0x7fffffff, mhi = 0;
+#pragma unroll
+  for (int j = 0; j < BKK_CHUNK; ++j) {
+    slo[j] = j < ns ? sh.lo[j] : 1;
+    shi[j] = j < ns ? sh.hi[j] : 0;
+    if (j < ns) {
+      mlo = min(mlo, slo[j]);
+      mhi = max(mhi, shi[j]);
+    }
+  }
+  double acc[BKK_CHUNK][4];
+#pragma unroll
+  for (int j = 0; j < BKK_CHUNK; ++j)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[j][t] = 0.0;
+  const int h = n / 2;
+  const int64_t nn = (int64_t)n * n;
+  const int64_t total = (int64_t)(kz_lim + 1) * nn;
+  for (int64_t i = (int64_t)blockIdx.x * BKK_NT + threadIdx.x; i < total; i += (int64_t)gridDim.x * BKK_NT) {
+    const int kz = (int)(i / nn);
+    const int r = (int)(i - (int64_t)kz * nn);
+    const int kx = r / n, ky = r - kx * n;
+    const int sx = kx <= h ? kx : kx - n, sy = ky <= h ? ky : ky - n;
+    const int m = sx * sx + sy * sy + kz * kz;
+    if (m < mlo || m >= mhi) continue;
+    const float2 d = dspec[i];
+    const float2 p = pspec[i];
+    const float w = wt[kx] * wt[ky] * wt[kz];
+    const double wh = (kz == 0 || kz == h) ? 1.0 : 2.0;
+    const double dx = (double)(d.x * w), dy = (double)(d.y * w);
+    const double d2 = wh * (dx * dx + dy * dy);
+    const double tr = wh * (dx * (double)p.x + dy * (double)p.y);
+    const double qq = qspec ? wh * (double)qspec[i].x : 0.0;
+#pragma unroll
+    for (int j = 0; j < BKK_CHUNK; ++j)
+      if (m >= slo[j] && m < shi[j]) {
+        acc[j][0] += d2;
+        acc[j][1] += wh;
+        acc[j][2] += tr;
+        acc[j][3] += qq;
+      }
+  }
+  const double inv = 1.0 / ((double)n * (double)n * (double)n);
+#pragma unroll
+  for (int j = 0; j < BKK_CHUNK; ++j)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const double v = block_sum_f64_waves<BKK_NT>(acc[j][t], red);
+      __syncthreads();
+      if (j < ns && threadIdx.x == 0)
+        a.part[((size_t)(s0 + j) * BK_SLOTS + t) * BK_PART_STRIDE + blockIdx.x] = v * inv;
+    }
+}
+
 constexpr int BKF_NT = 256;
 __global__ void __launch_bounds__(BKF_NT) k_bispec_finalize(BispecArgs a, int n, int g_pair, int g_red) {
   __shared__ double red[BKF_NT / 64 + 1];
@@ -1278,10 +1271,10 @@ template <int N> struct FFTLaunch {
     hipLaunchKernelGGL((k_xpass_inverse<N, SRC>), dim3(gx), dim3(Cf::NTX), 0, s, in, scratch, tw, g, sm, kz_lim);
     return check_hip(hipGetLastError(), "inverse x pass launch");
   }
-  static int red_grid() {
-    static int g = 0;
-    if (!g) g = cmin(BK_PART_STRIDE, persistent_grid(k_zpass_c2r_reduce<N>, Cf::NTZ, ZTILES));
-    return g;
+  static int kspace_grid() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return cmin(BK_PART_STRIDE, num_cus(dev) * 8);
   }
   static int pair_grid() {
     static int g = 0;
@@ -1312,8 +1305,15 @@ template <int N> struct FFTLaunch {
     if (kl >= 0) hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw, kl);
     return check_hip(hipGetLastError(), "shell inverse launch");
   }
-  static int bispec(const float* mesh, float2* spec, float2* scratch, float* const* meshes, const float2* tw,
-                    const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s) {
+  static int r2c_into(const float* mesh, float2* out, const float2* tw, hipStream_t s) {
+    int e = zpass(mesh, out, tw, nullptr, nullptr, s);
+    if (!e) e = ypass(out, tw, s);
+    if (!e) e = xpass(out, tw, s);
+    return e;
+  }
+  static int bispec(const float* mesh, float2* spec, float2* scratch, float2* qspec, float* const* meshes,
+                    const float2* tw, const float* wt, const int* lo, const int* hi, const BispecArgs& a,
+                    hipStream_t s) {
     static int gz = 0;
     if (!gz) gz = persistent_grid(k_zpass_c2r<N>, Cf::NTZ, ZTILES);
     int e = zpass(mesh, spec, tw, nullptr, nullptr, s);
@@ -1337,14 +1337,26 @@ template <int N> struct FFTLaunch {
     if (!a.i_sums)
       hipLaunchKernelGGL(k_bispec_pair, dim3(gp), dim3(PAIR_NT), 0, s, meshes[2], (const float*)meshes[3], total,
                          slot(0, 1), slot(1, 1));
-    const int gr = red_grid();
-    for (int sh = 2; sh < a.nshell; ++sh)
-      for (int one = 0; one < nones; ++one) {
-        e = shell_inverse(one != 0, spec, scratch, tw, ShellMask{lo[sh], hi[sh], wt}, s);
-        if (e) return e;
-        hipLaunchKernelGGL(k_zpass_c2r_reduce<N>, dim3(gr), dim3(Cf::NTZ), 0, s, (const float2*)scratch, tw,
-                           (const float*)meshes[one * 2], slot(sh, one), slot(sh, 2 + one), shell_kz_lim(hi[sh]));
+    // later shells in Fourier space against the spectra of the product meshes
+    int gr = 1;
+    if (a.nshell > 2) {
+      e = r2c_into(meshes[0], scratch, tw, s);
+      if (!e && !a.i_sums) e = r2c_into(meshes[2], qspec, tw, s);
+      if (e) return e;
+      int kl = -1;
+      for (int sh = 2; sh < a.nshell; ++sh) kl = kl > shell_kz_lim(hi[sh]) ? kl : shell_kz_lim(hi[sh]);
+      gr = kspace_grid();
+      for (int s0 = 2; s0 < a.nshell; s0 += BKK_CHUNK) {
+        const int ns = a.nshell - s0 < BKK_CHUNK ? a.nshell - s0 : BKK_CHUNK;
+        ShellChunk ch{};
+        for (int j = 0; j < ns; ++j) {
+          ch.lo[j] = lo[s0 + j];
+          ch.hi[j] = hi[s0 + j];
+        }
+        hipLaunchKernelGGL(k_bispec_kspace, dim3(gr), dim3(BKK_NT), 0, s, (const float2*)spec, (const float2*)scratch,
+                           a.i_sums ? nullptr : (const float2*)qspec, wt, ch, s0, ns, N, kl, a);
       }
+    }
     hipLaunchKernelGGL(k_bispec_finalize, dim3(a.nshell), dim3(BKF_NT), 0, s, a, N, gp, gr);
     if (a.nshell > 2)
       hipLaunchKernelGGL(k_bispec_bq, dim3((a.nshell - 2 + 63) / 64), dim3(64), 0, s, a, N);
@@ -1534,10 +1546,10 @@ int xi_grid_x(int n) {
   return 0;
 }
 
-int fft_bispec(int n, const float* mesh, float2* spec, float2* scratch, float* const* meshes, const float2* tw,
-               const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s) {
+int fft_bispec(int n, const float* mesh, float2* spec, float2* scratch, float2* qspec, float* const* meshes,
+               const float2* tw, const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s) {
   switch (n) {
-#define PK_CASE(V) case V: return FFTLaunch<V>::bispec(mesh, spec, scratch, meshes, tw, wt, lo, hi, a, s);
+#define PK_CASE(V) case V: return FFTLaunch<V>::bispec(mesh, spec, scratch, qspec, meshes, tw, wt, lo, hi, a, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }

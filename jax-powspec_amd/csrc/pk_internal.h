@@ -200,10 +200,17 @@ struct BispecArgs {
   float* bk;             // [nshell - 2]
   float* qk;             // [nshell - 2]
 };
-// mesh -> delta_k (r2c into spec) -> per shell masked inverses (scratch) -> meshes / fused
-// reductions -> finalize; meshes m[4] = delta1, delta2, I1, I2 (n^3 float32 each)
-int fft_bispec(int n, const float* mesh, float2* spec, float2* scratch, float* const* meshes, const float2* tw,
-               const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s);
+// thresholds of up to 8 shells, passed by value to the Fourier-space shell sums
+struct ShellChunk {
+  int lo[8];
+  int hi[8];
+};
+// mesh -> delta_k (r2c into spec) -> the first two shells' fields (masked inverses through
+// scratch) and their products -> spectra of the products (scratch, qspec) -> later shells'
+// sums in Fourier space -> finalize; meshes m[4] = delta1, delta2, I1, I2 (n^3 float32 each);
+// lo / hi: host thresholds
+int fft_bispec(int n, const float* mesh, float2* spec, float2* scratch, float2* qspec, float* const* meshes,
+               const float2* tw, const float* wt, const int* lo, const int* hi, const BispecArgs& a, hipStream_t s);
 
 // ---- configuration-space multipoles (pk_fft.hip) -------------------------------------------
 // largest bin count of the separation binning (its 64-bit accumulators live in LDS)
