@@ -576,15 +576,18 @@ def bench_bispec(dev, reps=3, cpu=True):
     mesh = 4 * n ** 3
     _, _, _, hi = correlations.bispec_shells(n, box, k1, k2, theta)
     nsh = hi.size
-    # with reused indicator sums: forward transform of the field (mesh read, z write, y and x
-    # read + write); the first two shells' delta fields (plane-limited x write, y read +
-    # write, C2R read, mesh write); their product (two meshes read, one written); its forward
-    # transform; the later shells' Fourier-space sums (delta_k and the product spectrum read
-    # over the planes kz <= isqrt(hi - 1) of the largest shell, once per chunk of 8 shells)
+    # with reused indicator sums (a shell touches the planes kz <= isqrt(hi - 1) only):
+    # forward transform of the field up to the largest shell's plane (mesh read, z write,
+    # y and x read + write); the first two shells' delta fields (x write, y read + write,
+    # C2R read, mesh write); their product (two meshes read, one written); its forward
+    # transform up to the later shells' largest plane; the Fourier-space sums (delta_k and
+    # the product spectrum read over those planes, once per chunk of 8 shells)
     planes = [min(math.isqrt(int(h) - 1), n // 2) + 1 if h > 0 else 0 for h in hi]
+    plane = 8 * n * n
     chunks = (nsh - 2 + 7) // 8
-    byts = (2 * (mesh + 5 * half) + sum(4 * 8 * n * n * p + mesh for p in planes[:2]) + 3 * mesh
-            + chunks * 2 * 8 * n * n * max(planes[2:], default=0))
+    pmax, plate = max(planes), max(planes[2:], default=0)
+    byts = ((mesh + 5 * plane * pmax) + sum(4 * plane * p + mesh for p in planes[:2]) + 3 * mesh
+            + (mesh + 5 * plane * plate) + chunks * 2 * plane * plate)
     gbs = byts / (ms * 1e-3) / 1e9
     out = {"workload": f"bispec {n}^3, L={box:g}, k1={k1}, k2={k2}, {theta.size} angles ({nsh} shells)",
            "ms": round(ms, 2), "per_s": round(1e3 / ms, 2), "ms_first_call": round(ms_all, 2),

@@ -44,7 +44,7 @@ template <int N> struct FFTCfg {
 template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTZ) __attribute__((amdgpu_waves_per_eu(FFTCfg<N>::WPEZ)))
 k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2* __restrict__ twg,
-        const double* __restrict__ stats, double* __restrict__ dc_part) {
+        const double* __restrict__ stats, double* __restrict__ dc_part, int kz_lim) {
   using Cf = FFTCfg<N>;
   constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
   constexpr int YT = N / C;
@@ -86,7 +86,8 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
     __syncthreads();
     lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
     // split: X[k] = Fe + e^{-2 pi i k / N} Fo, Fe = (Z_k + Z*_{NC-k}) / 2, Fo = -i (Z_k - Z*_{NC-k}) / 2
-    for (int i = tid; i < (NC + 1) * C; i += NT) {
+    // output planes kz <= kz_lim only (callers that read no higher plane pass less than n/2)
+    for (int i = tid; i < (min(kz_lim, NC) + 1) * C; i += NT) {
       const int c = i % C, k = i / C;
       const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
       const float2 zr = buf[c * CS + (k == 0 ? 0 : NC - k)];
@@ -165,6 +166,12 @@ __global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass(float2* __restrict__ s
 }
 
 template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_lim(float2* __restrict__ spec,
+                                                              const float2* __restrict__ twg, int kz_lim) {
+  ypass_body<N>(spec, twg, blockIdx.x, gridDim.x, (kz_lim + 1) * N / FFTCfg<N>::CY);
+}
+
+template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_jobs(const Job* __restrict__ jobs,
                                                                const float2* __restrict__ twg) {
   const Job& j = jobs[blockIdx.y];
@@ -177,7 +184,7 @@ __global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_jobs(const Job* __rest
 // ------------------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTX)
-k_xpass(float2* __restrict__ spec, const float2* __restrict__ twg) {
+k_xpass(float2* __restrict__ spec, const float2* __restrict__ twg, int kz_lim) {
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
   constexpr int YT = N / C;
@@ -186,7 +193,8 @@ k_xpass(float2* __restrict__ spec, const float2* __restrict__ twg) {
   __shared__ float2 tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
-  for (int tile = blockIdx.x; tile < NTILES; tile += gridDim.x) {
+  const int ntiles = min(NTILES, (kz_lim + 1) * YT);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int kz = tile / YT, yt = tile % YT;
     float2* base = spec + (size_t)kz * N * N + (size_t)yt * C;
     stage_tile<N, C, PS, NT>(buf, base, N);
@@ -1189,10 +1197,10 @@ template <int N> struct FFTLaunch {
     return grid;
   }
   static int zpass(const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
-                   hipStream_t s) {
+                   hipStream_t s, int kz_lim = N / 2) {
     const int grid = zgrid();
     StageTimer t(ST_ZPASS, s);
-    hipLaunchKernelGGL(k_zpass<N>, dim3(grid), dim3(Cf::NTZ), 0, s, mesh, spec, tw, stats, dc_part);
+    hipLaunchKernelGGL(k_zpass<N>, dim3(grid), dim3(Cf::NTZ), 0, s, mesh, spec, tw, stats, dc_part, kz_lim);
     return check_hip(hipGetLastError(), "z pass launch");
   }
   static int ypass(float2* spec, const float2* tw, hipStream_t s) {
@@ -1208,11 +1216,11 @@ template <int N> struct FFTLaunch {
     hipLaunchKernelGGL(k_ypass_jobs<N>, dim3(gy, njobs), dim3(Cf::NTY), 0, s, jobs, tw);
     return check_hip(hipGetLastError(), "batched y pass launch");
   }
-  static int xpass(float2* spec, const float2* tw, hipStream_t s) {
+  static int xpass(float2* spec, const float2* tw, hipStream_t s, int kz_lim = N / 2) {
     static int grid = 0;
     if (!grid) grid = persistent_grid(k_xpass<N>, Cf::NTX, XTILES);
     StageTimer t(ST_XPASS_PLAIN, s);
-    hipLaunchKernelGGL(k_xpass<N>, dim3(grid), dim3(Cf::NTX), 0, s, spec, tw);
+    hipLaunchKernelGGL(k_xpass<N>, dim3(grid), dim3(Cf::NTX), 0, s, spec, tw, kz_lim);
     return check_hip(hipGetLastError(), "x pass launch");
   }
   // blocks per job of a batch of njobs: the resident capacity of the whole chip shared out,
@@ -1305,10 +1313,15 @@ template <int N> struct FFTLaunch {
     if (kl >= 0) hipLaunchKernelGGL(k_ypass_inverse<N>, dim3(gy), dim3(Cf::NTY), 0, s, scratch, tw, kl);
     return check_hip(hipGetLastError(), "shell inverse launch");
   }
-  static int r2c_into(const float* mesh, float2* out, const float2* tw, hipStream_t s) {
-    int e = zpass(mesh, out, tw, nullptr, nullptr, s);
-    if (!e) e = ypass(out, tw, s);
-    if (!e) e = xpass(out, tw, s);
+  // forward transform of the planes kz <= kz_lim only (higher planes are left unwritten)
+  static int r2c_into(const float* mesh, float2* out, const float2* tw, hipStream_t s, int kz_lim = N / 2) {
+    static int gy = 0;
+    if (!gy) gy = persistent_grid(k_ypass_lim<N>, Cf::NTY, YTILES);
+    int e = zpass(mesh, out, tw, nullptr, nullptr, s, kz_lim);
+    if (e) return e;
+    if (kz_lim >= 0) hipLaunchKernelGGL(k_ypass_lim<N>, dim3(gy), dim3(Cf::NTY), 0, s, out, tw, kz_lim);
+    e = check_hip(hipGetLastError(), "y pass launch");
+    if (!e) e = xpass(out, tw, s, kz_lim);
     return e;
   }
   static int bispec(const float* mesh, float2* spec, float2* scratch, float2* qspec, float* const* meshes,
@@ -1316,9 +1329,10 @@ template <int N> struct FFTLaunch {
                     hipStream_t s) {
     static int gz = 0;
     if (!gz) gz = persistent_grid(k_zpass_c2r<N>, Cf::NTZ, ZTILES);
-    int e = zpass(mesh, spec, tw, nullptr, nullptr, s);
-    if (!e) e = ypass(spec, tw, s);
-    if (!e) e = xpass(spec, tw, s);
+    // no shell reads a plane above the largest shell's limit
+    int klmax = -1;
+    for (int sh = 0; sh < a.nshell; ++sh) klmax = klmax > shell_kz_lim(hi[sh]) ? klmax : shell_kz_lim(hi[sh]);
+    int e = r2c_into(mesh, spec, tw, s, klmax);
     if (e) return e;
     // first two shells: stored fields (the indicator fields only when their sums are unknown)
     const int nones = a.i_sums ? 1 : 2;
@@ -1340,11 +1354,11 @@ template <int N> struct FFTLaunch {
     // later shells in Fourier space against the spectra of the product meshes
     int gr = 1;
     if (a.nshell > 2) {
-      e = r2c_into(meshes[0], scratch, tw, s);
-      if (!e && !a.i_sums) e = r2c_into(meshes[2], qspec, tw, s);
-      if (e) return e;
       int kl = -1;
       for (int sh = 2; sh < a.nshell; ++sh) kl = kl > shell_kz_lim(hi[sh]) ? kl : shell_kz_lim(hi[sh]);
+      e = r2c_into(meshes[0], scratch, tw, s, kl);
+      if (!e && !a.i_sums) e = r2c_into(meshes[2], qspec, tw, s, kl);
+      if (e) return e;
       gr = kspace_grid();
       for (int s0 = 2; s0 < a.nshell; s0 += BKK_CHUNK) {
         const int ns = a.nshell - s0 < BKK_CHUNK ? a.nshell - s0 : BKK_CHUNK;
