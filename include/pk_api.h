@@ -4,6 +4,9 @@
  * Path: particles -> CIC/TSC mass assignment -> density contrast -> 3D R2C FFT ->
  *       window deconvolution -> |delta_k|^2 k-shell binning with Legendre l = 0, 2, 4
  *       weights -> P_l(k) (+ shot-noise subtraction).
+ * On the same plans and transforms: configuration-space multipoles xi_l(s), the FFT
+ * bispectrum, and a seeded lognormal mock generator (Gaussian field, inverse FFT,
+ * Poisson population).
  *
  * The reference (dforero0896/jax-powspec) is pure Python/JAX and has no FFI; each entry
  * point below replaces the reference function named in its comment.  The Python drop-in
