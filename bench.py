@@ -182,15 +182,44 @@ def pmc_summary(fetch_csv, write_csv, out_path, command):
     print("\n".join(lines))
 
 
-def cpu_baseline(pos_host, n, box, edges, shot):
+def cpu_baseline(pos_host, n, box, edges, shot, reps=5):
+    """float32 CPU restatement of the reference (oracle/cpu_fp32.py) on the same catalogue:
+    one warm-up, then the median of `reps` timed estimates."""
     from oracle import cpu_fp32
     cores = cpu_fp32.ncores()
-    t0 = time.perf_counter()
-    cpu_fp32.estimate_f32(pos_host[:, 0], pos_host[:, 1], pos_host[:, 2], box, n, edges, shot, workers=cores)
-    dt = time.perf_counter() - t0
+    run = lambda: cpu_fp32.estimate_f32(pos_host[:, 0], pos_host[:, 1], pos_host[:, 2], box, n, edges, shot,
+                                        workers=cores)
+    run()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    dt = float(np.median(ts))
     return {"value": 1.0 / dt, "unit": "estimates/s", "cores": cores, "kind": "port",
-            "sample": f"1 full estimate of the same {n}^3 / {pos_host.shape[0]:.3g}-particle catalogue "
-                      f"({dt:.1f} s; FFT on {cores} threads, scatter and binning single-threaded NumPy)"}
+            "sample": f"median of {reps} full estimates (after 1 warm-up) of the same {n}^3 / "
+                      f"{pos_host.shape[0]:.3g}-particle catalogue ({dt:.1f} s each, range {min(ts):.1f}-"
+                      f"{max(ts):.1f} s; FFT on {cores} threads, scatter and binning single-threaded NumPy)"}
+
+
+def parity_vs_oracle(pos_host, n, box, edges, shot, sums, counts):
+    """Per-multipole error of the timed GPU estimate against the FP64 oracle on the same
+    float32 positions (checker only): eps_l = max_b |P_l - P_l^oracle| / A_l,b with A the
+    same-bin scale sum |L_l| P2 norm (2l+1) / N_b, and the median plain relative error."""
+    from oracle import ref_fp64 as R
+    t0 = time.perf_counter()
+    res = R.estimate(pos_host[:, 0], pos_host[:, 1], pos_host[:, 2], None, box, n, edges, stencil="CIC_REF", p=2)
+    opk, oex = np.asarray(res[1], np.float64), res[3]
+    cnt = np.asarray(counts)
+    P = np.asarray(sums) / cnt[:, None] * (box / n ** 2) ** 3 * np.array([1.0, 5.0, 9.0])
+    eps = np.abs(P - opk) / oex["scale"]
+    rel = np.abs(P - opk) / np.abs(opk)
+    return {"oracle": "FP64 NumPy restatement of the reference (oracle/ref_fp64.py), same float32 positions",
+            "eps_same_bin_scale": {f"P{2 * l}": float(eps[:, l].max()) for l in range(3)},
+            "rel_median": {f"P{2 * l}": float(np.median(rel[:, l])) for l in range(3)},
+            "rel_max": {f"P{2 * l}": float(rel[:, l].max()) for l in range(3)},
+            "gate": 1e-5, "counts_equal": bool(np.array_equal(cnt, oex["counts"])),
+            "oracle_s": round(time.perf_counter() - t0, 1)}
 
 
 def bench_small(dev, steps=50, cpu=True):
@@ -491,7 +520,7 @@ def bench_mock(dev, reps=5, cpu=True):
         dt = time.perf_counter() - t0
         scaled = dt * (n / ns) ** 3
         out["cpu_baseline"] = {"ms_scaled": round(scaled * 1e3, 1), "unit": "ms per mock", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", "extrapolated": True,
                                "sample": f"oracle NumPy pipeline at {ns}^3 (L={bs:g}, {p.shape[0]} particles, "
                                          f"{dt:.2f} s), per-cell time scaled to {n}^3"}
         out["gpu_over_cpu"] = round(scaled * 1e3 / out["real"]["ms"], 1)
@@ -539,7 +568,7 @@ def bench_xi(dev, reps=10, cpu=True):
         dt = time.perf_counter() - t0
         scaled = dt * (n / ns) ** 3 * math.log2(n) / math.log2(ns)
         out["cpu_baseline"] = {"ms_scaled": round(scaled * 1e3, 1), "unit": "ms per xi_vec", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", "extrapolated": True,
                                "sample": f"oracle FP64 NumPy xi_vec at {ns}^3 ({dt:.2f} s), scaled by n^3 log n "
                                          f"to {n}^3"}
         out["gpu_over_cpu"] = round(scaled * 1e3 / ms, 1)
@@ -605,7 +634,7 @@ def bench_bispec(dev, reps=3, cpu=True):
         dt = time.perf_counter() - t0
         scaled = dt * (n / ns) ** 3 * math.log2(n) / math.log2(ns)
         out["cpu_baseline"] = {"ms_scaled": round(scaled * 1e3, 1), "unit": "ms per bispec", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", "extrapolated": True,
                                "sample": f"oracle FP64 NumPy bispec at {ns}^3 ({dt:.2f} s), scaled by n^3 log n "
                                          f"to {n}^3"}
         out["gpu_over_cpu"] = round(scaled * 1e3 / ms, 1)
@@ -721,6 +750,8 @@ def main():
             if nl[i]:
                 stages[lib.pk_stage_name(i).decode()] = {"avg_ms": msv[i] / nl[i], "launches": int(nl[i])}
     pk = est.out.pk.cpu().numpy()
+    gpu_sums = est.out.sums.cpu().numpy()
+    gpu_counts = est.out.counts.cpu().numpy()
     log(f"P_0 first bins: {pk[:4, 0]}")
     roof = None
     if stages:
@@ -770,10 +801,15 @@ def main():
         bis = bench_bispec(dev, cpu=not args.no_cpu)
         log(f"bispec: {bis}")
         torch.cuda.empty_cache()
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(pos.cpu().numpy(), N_MESH, BOX, edges, shot)
+        pos_h = pos.cpu().numpy()
+        cpu = cpu_baseline(pos_h, N_MESH, BOX, edges, shot)
+        log(f"cpu baseline: {cpu}")
+        log("parity against the FP64 oracle ...")
+        parity = parity_vs_oracle(pos_h, N_MESH, BOX, edges, shot, gpu_sums, gpu_counts)
+        log(f"parity: {parity}")
     out = {
         "metric": "P_l(k) estimates/sec at 512^3, 2e7 particles; batched/served vs plain; p99 ms",
         "value": round(1e3 / ms, 3),
@@ -791,6 +827,7 @@ def main():
                                "P_0,2,4 in 256 k-bins", "mesh": N_MESH, "particles": npart, "bins": NBINS},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "parity": parity,
         "small": small,
         "batched": batched,
         "served": served,

@@ -37,6 +37,9 @@
  *     (kx, ky in [0, n), kz in [0, n/2], each once, k = 0 included when its bin exists) and
  *     FP64 ksum[nb] = sum of |k| over them in units of k_F (needed with PK_FLAG_KMEAN only).
  *     They depend on n and thr only and are exact (jax_powspec_amd.binning.bin_modes).
+ *     The per-bin counts an estimate returns are not copied from modes: the binning pass
+ *     counts the entries it bins, on the same pass as the sums (src/correlations.py:48), and
+ *     they normalise P_l; modes is read only for the mean |k| (ksum / count).
  */
 #ifndef PK_API_H
 #define PK_API_H
@@ -120,7 +123,8 @@ int pk_r2c(pk_plan* plan, const float* mesh, const double* stats, void* spectrum
  * Runs the z/y FFT passes into `spectrum` (caller buffer, (n/2+1) n^2 float2), then the
  * x pass fused with deconvolution (x / sin x)^(2p) per axis, |delta_k|^2, Legendre
  * weights and FP64 per-bin sums.  Outputs (device, nb entries; any may be NULL except
- * pk): sums[nb][3] FP64 (S0, S2, S4), counts[nb] int64 (a copy of modes), pk[nb][3] float32
+ * pk): sums[nb][3] FP64 (S0, S2, S4), counts[nb] int64 (stored modes the pass binned, the
+ * k = 0 mode included when its bin exists), pk[nb][3] float32
  * = S_l / N * (2l+1) * (box/n^2)^3 with P_0 -= shot_noise (NaN for empty bins),
  * nmodes[nb] float32, kmean[nb] FP64 (needs PK_FLAG_KMEAN; units of k_F).
  */

@@ -159,7 +159,7 @@ int make_args(ScatterArgs* a, const float* x, int64_t sx, const float* y, int64_
 // workspace of pk_powspec_mesh: bin partials + z-pass DC partials (spectrum is an argument)
 size_t powspec_ws_bytes(int n, int nb) {
   const int G = bin_grid(n, nb);
-  return align_up((size_t)G * (size_t)(nb > 0 ? nb : 1) * 3 * sizeof(double)) +
+  return align_up((size_t)G * (size_t)(nb > 0 ? nb : 1) * BIN_PART_LEN * sizeof(double)) +
          align_up((size_t)zpass_grid(n) * sizeof(double));
 }
 
@@ -208,7 +208,7 @@ int finish_powspec(pk_plan* p, const Job& j, hipStream_t s) {
 
 double* dc_partials(void* ws, int n, int nb) {
   return reinterpret_cast<double*>(static_cast<char*>(ws) +
-                                   align_up((size_t)bin_grid(n, nb) * (size_t)nb * 3 * sizeof(double)));
+                                   align_up((size_t)bin_grid(n, nb) * (size_t)nb * BIN_PART_LEN * sizeof(double)));
 }
 
 int run_powspec(pk_plan* p, const float* mesh, const double* stats, const int32_t* thr, const int32_t* jstart,
@@ -595,7 +595,7 @@ SlotLayout slot_layout(int n, int64_t max_np, int max_nb, const TileGeom& cap, i
   L.spec = 0;
   size_t off = align_up((size_t)(n / 2 + 1) * n * n * sizeof(float2));
   L.partials = off;
-  off += align_up((size_t)gmax * (size_t)max_nb * 3 * sizeof(double));
+  off += align_up((size_t)gmax * (size_t)max_nb * BIN_PART_LEN * sizeof(double));
   L.scatter = off;
   Job cj{};
   cj.g = cap;

@@ -234,43 +234,48 @@ constexpr int DPP_WAVE_SHR1 = 0x138; // lane l reads lane l - 1
 constexpr int DPP_WAVE_SHL1 = 0x130; // lane l reads lane l + 1
 
 // one step of a segmented inclusive scan: (v, f) <- (v + v_src, f_src) where this lane's
-// run has not started yet (f == 0); a lane without a source reads (0, 0) and is unchanged
+// run has not started yet (f == 0); a lane without a source reads (0, 0) and is unchanged.
+// fc packs an integer count with the flag, fc = 2 cnt + f: while f == 0 its low bit is 0, so
+// fc + fc_src = 2 (cnt + cnt_src) + f_src carries both in one move and one add.
 template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step(float& a, float& b, float& c, int& f) {
+__device__ __forceinline__ void seg_step(float& a, float& b, float& c, int& fc) {
   const float pa = dpp_move<CTRL, ROWS>(a), pb = dpp_move<CTRL, ROWS>(b), pc = dpp_move<CTRL, ROWS>(c);
-  const int pf = dpp_move<CTRL, ROWS>(f);
-  if (!f) {
+  const int pfc = dpp_move<CTRL, ROWS>(fc);
+  if (!(fc & 1)) {
     a += pa;
     b += pb;
     c += pc;
-    f = pf;
+    fc += pfc;
   }
 }
 
-// Sums of the values over each run of lanes with equal `slot` inside a row of 16 lanes
-// (lane order), left in the run's last lane of the row; returns true in that lane.  A
-// segmented inclusive scan with shifts 1, 2, 4, 8 inside rows of 16: a run that crosses a
-// row boundary is flushed once per row, which costs an extra histogram update but saves
-// the two cross-row steps (row_bcast15 into rows 1 and 3, then row_bcast31 into rows 2
-// and 3) of a full-wave scan.  Float partial sums of at most 16 terms (depth-4 tree:
-// relative error <= 4 ulp of the run's absolute sum).  Must be called by all 64 lanes.
-__device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, float& c) {
+// Sums of the values (and of the integer cnt) over each run of lanes with equal `slot`
+// inside a row of 16 lanes (lane order), left in the run's last lane of the row; returns
+// true in that lane.  A segmented inclusive scan with shifts 1, 2, 4, 8 inside rows of 16:
+// a run that crosses a row boundary is flushed once per row, which costs an extra
+// histogram update but saves the two cross-row steps (row_bcast15 into rows 1 and 3, then
+// row_bcast31 into rows 2 and 3) of a full-wave scan.  Float partial sums of at most 16
+// terms (depth-4 tree: relative error <= 4 ulp of the run's absolute sum); cnt is exact
+// (0 <= cnt, run total < 2^30).  Must be called by all 64 lanes.
+__device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, float& c, int& cnt) {
   const int lane = threadIdx.x & 63;
   // every cross-lane move runs with all 64 lanes active (a lane the exec mask leaves out
   // reads as 0 to its neighbours), so none sits behind a short-circuit or a branch
   const int prev = dpp_move<DPP_WAVE_SHR1>(slot + 1) - 1;   // lane 0 reads -1 (no source)
-  int f = (lane == 0 || prev != slot) ? 1 : 0;               // this lane starts a run
+  const int f = (lane == 0 || prev != slot) ? 1 : 0;         // this lane starts a run
   const int next_starts = dpp_move<DPP_WAVE_SHL1>(f);
   const bool tail = ((lane & 15) == 15) | (next_starts != 0);
-  seg_step<DPP_ROW_SHR + 1, 0xf>(a, b, c, f);
-  seg_step<DPP_ROW_SHR + 2, 0xf>(a, b, c, f);
-  seg_step<DPP_ROW_SHR + 4, 0xf>(a, b, c, f);
-  seg_step<DPP_ROW_SHR + 8, 0xf>(a, b, c, f);
+  int fc = 2 * cnt + f;
+  seg_step<DPP_ROW_SHR + 1, 0xf>(a, b, c, fc);
+  seg_step<DPP_ROW_SHR + 2, 0xf>(a, b, c, fc);
+  seg_step<DPP_ROW_SHR + 4, 0xf>(a, b, c, fc);
+  seg_step<DPP_ROW_SHR + 8, 0xf>(a, b, c, fc);
+  cnt = fc >> 1;
   return tail;
 }
 
 constexpr int HB_LDS = 256;   // bins held in LDS; above this the per-block slice lives in HBM
-constexpr int NACC = 3;       // S0, S2, S4 (mode counts and sums of |k| come from the host tables)
+constexpr int NACC = BIN_PART_LEN;   // S0, S2, S4 and the binned mode count per block and bin
 
 // ------------------------------------------------------------------------------------
 // x pass fused with deconvolution, |delta_k|^2, Legendre weights and per-bin FP64 sums.
@@ -283,7 +288,8 @@ template <int N> struct XBinLds {
   static constexpr size_t TILE = (size_t)N * (FFTCfg<N>::CX + 1) * 8;
   static constexpr int JLEN = isqrt_c(3 * (N / 2) * (N / 2)) + 2;   // isqrt-start table entries
   static constexpr size_t HIST =
-      (size_t)(HB_LDS + 1) * 4 + 3 * HB_LDS * 8 + (size_t)(N / 2 + 1) * 4 + (size_t)JLEN * 2 + (size_t)(N / 4) * 8;
+      (size_t)(HB_LDS + 1) * 4 + 3 * HB_LDS * 8 + HB_LDS * 4 + (size_t)(N / 2 + 1) * 4 + (size_t)JLEN * 2 +
+      (size_t)(N / 4) * 8;
   // LDS-limited waves per SIMD (the histogram in HBM: no limit is imposed)
   template <bool LH> static constexpr int wpe() {
     return LH ? cmax(1, cmin(4, (int)((160 * 1024) / (TILE + HIST)) * (FFTCfg<N>::NTX / 64) / 4)) : 1;
@@ -309,6 +315,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   __shared__ float2 buf[N * PS];
   __shared__ int thr_s[LH ? HB_LDS + 1 : 1];
   __shared__ double hs[LH ? 3 * HB_LDS : 1];
+  __shared__ unsigned hc[LH ? HB_LDS : 1];   // stored modes binned per bin
   __shared__ float wt_s[LH ? H + 1 : 1];
   __shared__ short jst_s[LH ? JLEN : 1];   // bins < HB_LDS
   __shared__ float2 twq[N / 4];
@@ -319,6 +326,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
     for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0.0;
+    for (int i = tid; i < HB_LDS; i += NT) hc[i] = 0u;
     for (int i = tid; i <= H; i += NT) wt_s[i] = wtab[i];
     for (int i = tid; i < JLEN; i += NT) jst_s[i] = (short)jst_g[i];
   } else {
@@ -454,7 +462,11 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
         const float2 d2 = buf[qm * PS + cB];
         p2 += fB * ((c2.x * c2.x + c2.y * c2.y) + pair * (d2.x * d2.x + d2.y * d2.y));
       }
-      p2 = (slot >= 0 && m > 0) ? p2 * (wqi * wyz) : 0.0f;
+      const bool live = slot >= 0 && m > 0;
+      p2 = live ? p2 * (wqi * wyz) : 0.0f;
+      // stored entries this task bins: (q, ky) and (N - q, ky) when paired, times two for a
+      // quad; the k = 0 mode is counted by the finalize
+      int cnt = live ? (quad ? 2 : 1) * (q != 0 && q != H ? 2 : 1) : 0;
       // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
       //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
       // so L2 is exactly 0 where 3 kz^2 = m; 1 / m to within 1 ulp (hardware reciprocal)
@@ -464,17 +476,19 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       const float l4 = 0.125f * (float)num4 * rm * rm;
       float v2 = p2 * l2, v4 = p2 * l4;
       // runs of equal slot in lane order -> one histogram update per run (its last lane)
-      const bool tail = wave_run_sums(slot, p2, v2, v4);
+      const bool tail = wave_run_sums(slot, p2, v2, v4, cnt);
       if (!tail || slot < 0) continue;
       if constexpr (LH) {
         atomicAdd(&hs[slot], (double)p2);
         atomicAdd(&hs[HB_LDS + slot], (double)v2);
         atomicAdd(&hs[2 * HB_LDS + slot], (double)v4);
+        atomicAdd(&hc[slot], (unsigned)cnt);
       } else {
         double* g = gpart + (size_t)slot * NACC;
         atomicAdd(g + 0, (double)p2);
         atomicAdd(g + 1, (double)v2);
         atomicAdd(g + 2, (double)v4);
+        atomicAdd(g + 3, (double)cnt);   // exact: integers below 2^53
       }
     }
     __syncthreads();
@@ -485,6 +499,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       g[0] = hs[i];
       g[1] = hs[HB_LDS + i];
       g[2] = hs[2 * HB_LDS + i];
+      g[3] = (double)hc[i];
     }
   }
 }
@@ -522,7 +537,7 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   float* __restrict__ pk = j.pk;
   float* __restrict__ nmodes = j.nmodes;
   double* __restrict__ kmean = (j.flags & PK_FLAG_KMEAN) ? j.kmean : nullptr;
-  double part[NACC] = {0.0, 0.0, 0.0};
+  double part[NACC] = {0.0, 0.0, 0.0, 0.0};
   for (int g = threadIdx.x; g < G; g += FIN_NT) {
     const double* p = partials + ((size_t)g * nb + b) * NACC;
 #pragma unroll
@@ -537,8 +552,10 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   if (threadIdx.x != 0) return;
   // spectrum of rho - c (fused assignment): delta_k = rho_k * n^3 / M for k != 0
   const double sc = scale_stats ? scale_stats[1] * scale_stats[1] : 1.0;
-  // k = 0 lands in the last bin with thr[b] <= 0 (thresholds are non-decreasing), if kept
-  if (dc_part != nullptr && thr[b] <= 0 && (b + 1 == nb || thr[b + 1] > 0) && thr[nb] > 0) {
+  // k = 0 lands in the last bin with thr[b] <= 0 (thresholds are non-decreasing), if kept;
+  // the x pass skips it, so its count and (unless the contrast zeroed it) power enter here
+  const bool dc_bin = thr[b] <= 0 && (b + 1 == nb || thr[b + 1] > 0) && thr[nb] > 0;
+  if (dc_part != nullptr && dc_bin) {
     const double p0 = dc * dc;   // mu = 0: L2 = -1/2, L4 = 3/8
     acc[0] += p0;
     acc[1] += -0.5 * p0;
@@ -547,7 +564,9 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   acc[0] *= sc;
   acc[1] *= sc;
   acc[2] *= sc;
-  const int64_t nmode = j.modes[b];   // exact lattice count from the host tables
+  // stored modes the x pass binned (integer-valued partials, summed exactly), as the
+  // reference's unit-weight histogram on the same pass (src/correlations.py:48)
+  const int64_t nmode = (int64_t)acc[3] + (dc_bin ? 1 : 0);
   const double cnt = (double)nmode;
   if (sums) {
     sums[b * 3 + 0] = acc[0];

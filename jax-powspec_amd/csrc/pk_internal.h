@@ -84,6 +84,9 @@ struct Job {
 };
 
 // ---- FFT passes and binning (pk_fft.hip) -------------------------------------------------
+// per-block partials of the binning x pass: [G][nb][BIN_PART_LEN] doubles holding S_0, S_2,
+// S_4 and the number of stored modes the block binned (an exact integer)
+constexpr int BIN_PART_LEN = 4;
 bool fft_supported(int n);
 int zpass_grid(int n);
 int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
