@@ -136,6 +136,24 @@ int pk_powspec_mesh(pk_plan* plan, const float* mesh, const double* stats,
                     double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* --- P_l(k) from a mesh without allocation ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * computation and outputs as pk_powspec_mesh, with the workspace replaced by a fixed
+ * scratch array of PK_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs
+ * on until the process ends (its address is resolved when a plan is created).  The call
+ * allocates nothing; the caller still passes the spectrum and output buffers.  Fails with
+ * PK_E_WORKSPACE when pk_powspec_workspace_size(plan, nb) exceeds the scratch (large nb).
+ * Calls that overlap on one device share the scratch and race: serialise them (one stream,
+ * or events).
+ */
+#define PK_NOALLOC_SCRATCH_BYTES (16u << 20)
+int pk_powspec_mesh_noalloc(pk_plan* plan, const float* mesh, const double* stats,
+                            const int32_t* thr, const int32_t* jstart, const int64_t* modes, const double* ksum,
+                            int nb, int window_p, int flags,
+                            double box_size, double shot_noise, void* spectrum,
+                            double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean, void* stream);
+
 /* --- whole estimate -----------------------------------------------------------------
  * particles -> assignment (mesh_in = zeros) -> contrast folded into the FFT read ->
  * P_l(k).  Workspace holds the mesh, the spectrum and all scratch.

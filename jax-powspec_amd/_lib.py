@@ -24,6 +24,7 @@ FLAG_ZERO_DC = 1
 FLAG_KMEAN = 2
 STATS_LEN = 4
 NUM_STAGES = 14
+NOALLOC_SCRATCH_BYTES = 16 << 20
 
 # (name, restype, argtypes) for every symbol declared in include/pk_api.h
 _vp = C.c_void_p
@@ -67,6 +68,8 @@ SIGNATURES = [
     ("pk_powspec_workspace_size", _i, [_vp, _i, _psz]),
     ("pk_powspec_mesh", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp,
                              _vp, _vp, _sz, _vp]),
+    ("pk_powspec_mesh_noalloc", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _vp, _vp, _vp, _vp,
+                                     _vp, _vp, _vp]),
     ("pk_estimate_workspace_size", _i, [_vp, _i64, _i, _vp, _i, _psz]),
     ("pk_estimate", _i, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _f, _f, _f, _d, _i, _i, _vp,
                          _vp, _vp, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),

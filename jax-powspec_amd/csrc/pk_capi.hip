@@ -119,6 +119,38 @@ int host_plan_tables(int n, std::vector<float2>& tw, std::vector<float>& wt) {
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Fixed scratch of pk_powspec_mesh_noalloc: the bin partials and z-pass DC partials that
+// pk_powspec_mesh keeps in its workspace.  Every device holds its own copy from module
+// load until the process ends; every kernel writes the part it reads before reading it,
+// so nothing needs clearing between calls.  Calls that overlap on one device share it and
+// race.  Host code never passes `scratch` itself to a kernel (on the host that name is a
+// stand-in, not a device address): the address comes from hipGetSymbolAddress, resolved
+// once per device when a plan is created on it.
+static __device__ __align__(256) unsigned char scratch[PK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// resolves this device's scratch address (the device must be current); once per device
+int resolve_noalloc_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return set_error(PK_E_INVALID, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PK_OK;
+  void* ptr = nullptr;
+  int e = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch symbol address");
+  if (e) return e;
+  if (!ptr) return set_error(PK_E_LAUNCH, "scratch symbol address is null on device %d", device);
+  g_scratch_ptr[device] = ptr;
+  return PK_OK;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 int check_n(int n) {
   if (!fft_supported(n)) return set_error(PK_E_INVALID, "unsupported mesh side n=%d (need 8..1024, 2^a or 3*2^a)", n);
   return PK_OK;
@@ -266,6 +298,7 @@ int pk_plan_create(int n, pk_plan** out) {
   }
   e = check_hip(hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice), "plan upload");
   if (!e) e = check_hip(hipMemcpy(p->wtab, wt.data(), wt.size() * sizeof(float), hipMemcpyHostToDevice), "plan upload");
+  if (!e) e = resolve_noalloc_scratch(p->device);   // set-up of pk_powspec_mesh_noalloc's scratch
   if (e) {
     (void)hipFree(p->tw);
     (void)hipFree(p->wtab);
@@ -348,6 +381,28 @@ int pk_powspec_mesh(pk_plan* p, const float* mesh, const double* stats, const in
   return run_powspec(p, mesh, stats, thr, jstart, modes, ksum, nb, window_p, flags, box_size, shot_noise,
                      static_cast<float2*>(spectrum), sums, counts, pk, nmodes, kmean, workspace, workspace_bytes,
                      as_stream(stream));
+}
+
+int pk_powspec_mesh_noalloc(pk_plan* p, const float* mesh, const double* stats, const int32_t* thr,
+                            const int32_t* jstart, const int64_t* modes, const double* ksum, int nb, int window_p,
+                            int flags, double box_size, double shot_noise, void* spectrum, double* sums,
+                            int64_t* counts, float* pk, float* nmodes, double* kmean, void* stream) {
+  int e = check_plan(p);
+  if (e) return e;
+  e = check_bins(thr, jstart, modes, ksum, nb, flags, pk);
+  if (e) return e;
+  if (!mesh || !spectrum) return set_error(PK_E_INVALID, "null mesh / spectrum");
+  if (!(box_size > 0.0)) return set_error(PK_E_INVALID, "box_size must be > 0");
+  const size_t need = powspec_ws_bytes(p->n, nb);
+  if (need > PK_NOALLOC_SCRATCH_BYTES)
+    return set_error(PK_E_WORKSPACE,
+                     "pk_powspec_mesh_noalloc: n=%d with %d bins needs %zu bytes of scratch, above the fixed %zu "
+                     "(use pk_powspec_mesh with a workspace)",
+                     p->n, nb, need, (size_t)PK_NOALLOC_SCRATCH_BYTES);
+  void* ws = noalloc_scratch(p->device);
+  if (!ws) return set_error(PK_E_LAUNCH, "pk_powspec_mesh_noalloc: scratch address of device %d unknown", p->device);
+  return run_powspec(p, mesh, stats, thr, jstart, modes, ksum, nb, window_p, flags, box_size, shot_noise,
+                     static_cast<float2*>(spectrum), sums, counts, pk, nmodes, kmean, ws, need, as_stream(stream));
 }
 
 namespace {
