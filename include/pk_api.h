@@ -97,8 +97,10 @@ int pk_supported_n(int n);   /* 1 if n is supported, else 0 */
  * alias mesh_in.  Positions are element-strided (stride 3 reads an (N_p, 3) array in
  * place); w may be NULL (unit weights).  tile[3] = (1, ty, n) row-bundle tile of the sorted
  * LDS-privatised deposit (ty rows of full z extent; tile[2] may be 0 for n); tile == NULL
- * picks the default (ty = largest divisor of n that is <= 32); tile[0] == 0 selects direct
- * global atomics.  stats (nullable) receives the deposited mass (PK_STATS_LEN doubles).
+ * picks the default (ty = largest divisor of n that is <= 32 and keeps ty n 8 bytes within
+ * 128 KB: 32 up to n = 512, 16 at 768 and 1024); tile[0] == 0 selects direct global
+ * atomics.  Every n uses the sorted deposit unless direct is asked for: when the sort's
+ * per-key tables exceed LDS they live in HBM.  stats (nullable) receives the deposited mass (PK_STATS_LEN doubles).
  * Index/fraction arithmetic follows the reference in float32: inv = 1 / ((float)box /
  * (float)n), pos = (x - xmin) * inv.
  */
@@ -204,7 +206,7 @@ int pk_estimate_backward(pk_plan* plan,
  * graph on the first run of each job-count shape and replayed afterwards.
  *   - all jobs use the sorted + fused assignment path (tile == NULL: default rows per
  *     tile; tile = (1, ty, 0) with a compiled fused variant); pk_batch_create fails with
- *     PK_E_UNSUPPORTED when n has no such path (n >= 768);
+ *     PK_E_UNSUPPORTED when the tile has no compiled fused variant;
  *   - pk_batch_set_jobs copies the host job array into pinned memory and enqueues its
  *     upload on `stream`; the particle and output pointers must stay valid until the
  *     runs that use them are complete;

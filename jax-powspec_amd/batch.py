@@ -266,7 +266,8 @@ def powspec_batched(catalogues, *, tile="auto", graph: bool = True, device=None,
     box_size, n_bins and optional mas ('CIC_REF' default, 'CIC', 'TSC'), k_edges (None:
     fundamental bins), shot_noise, window, wrap, xmin/ymin/zmin.  tile: 'auto' (autotuned
     per mesh side on its own catalogues, cached), None (default rows per tile) or
-    (1, ty, 0).  Mesh sides without a batched path (n >= 768) raise ValueError.
+    (1, ty, 0).  Every supported mesh side has a batched path (at 768 and 1024 with 16-row
+    tiles and the sort's per-key tables in HBM).
     """
     device = _device.require_gpu(device)
     cats = list(catalogues)

@@ -628,7 +628,8 @@ bool timing_suspended() { return t_capturing; }
 
 namespace {
 bool same_dims(const BatchDims& a, const BatchDims& b) {
-  return a.njobs == b.njobs && a.max_nblk == b.max_nblk && a.max_nkeys == b.max_nkeys && a.max_nb == b.max_nb;
+  return a.njobs == b.njobs && a.max_nblk == b.max_nblk && a.max_nkeys == b.max_nkeys && a.max_nb == b.max_nb &&
+         a.lds_keys == b.lds_keys;
 }
 
 int batch_geometry(int n, int64_t max_np, const int* tile, TileGeom* cap) {
@@ -765,7 +766,10 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     b->pending[k] = false;
   }
   Job* h = b->h_jobs[k];
-  BatchDims d{njobs, 1, 1, 1};
+  BatchDims d{njobs, 1, 1, 1, 0};
+  // capacity grids: one LDS table size for every refill (the capacity geometry's keys if
+  // they fit, else none), so captured launches repeat; jobs with more keys use HBM tables
+  const int cap_lds_keys = (size_t)b->cap.nkeys <= lds_table_keys() ? b->cap.nkeys : 0;
   const int tile[3] = {1, b->cap.ty, 0};
   // counting chunk of a batch: about four rounds of blocks over the whole GPU for all its
   // particles (jobs of very different sizes balance), never more blocks per job than its slot holds
@@ -806,10 +810,16 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     if (j.g.nblk > d.max_nblk) d.max_nblk = j.g.nblk;
     if (j.g.nkeys > d.max_nkeys) d.max_nkeys = j.g.nkeys;
     if (q.nb > d.max_nb) d.max_nb = q.nb;
+    if (b->flags & PK_BATCH_CAPACITY_GRIDS) {
+      if (j.g.nkeys > cap_lds_keys) h[i].g.gtab = 1;
+    } else if (!j.g.gtab && j.g.nkeys > d.lds_keys) {
+      d.lds_keys = j.g.nkeys;
+    }
   }
   if (b->flags & PK_BATCH_CAPACITY_GRIDS) {
     d.max_nblk = b->cap.nblk;
     d.max_nkeys = b->cap.nkeys;
+    d.lds_keys = cap_lds_keys;
   }
   hipStream_t s = as_stream(stream);
   if (njobs > 0) {

@@ -16,29 +16,35 @@
 
 namespace pk {
 
-// Row-bundle height of the default tile (largest divisor of N that is <= 32, the same rule
-// as make_tile_geom) and of the second compiled variant (16 rows, N >= 64): the autotuner
-// picks between them.
+// Row-bundle height of the default tile (the rule of make_tile_geom: the largest divisor of
+// N that is <= 32 and whose rows fit the 64-bit fixed-point mesh deposit, so 16 at N = 768
+// and 1024) and of the second compiled variant (half of it): the autotuner picks between
+// them.
 template <int N> struct FusedTY {
-  static constexpr int DEF = N <= 32 ? N : (N % 32 == 0 ? 32 : (N % 24 == 0 ? 24 : 16));
-  static constexpr int ALT = (N >= 64 && N % 16 == 0 && DEF != 16) ? 16 : DEF;
+  static constexpr int D32 = N <= 32 ? N : (N % 32 == 0 ? 32 : (N % 24 == 0 ? 24 : 16));
+  static constexpr int DEF = (size_t)D32 * N * 8 <= FIXED_LDS_MAX ? D32 : 16;
+  static constexpr int ALT = (N >= 64 && DEF % 2 == 0 && N % (DEF / 2) == 0) ? DEF / 2 : DEF;
 };
 
+// The tile's cells are 64-bit fixed-point integers [TY][N]; after the deposit they are
+// converted to float rows [TY][N + 2] (the z transform's complex rows of N/2 + 1) in the
+// same LDS.
 template <int N, int TY_> struct FusedCfg {
   static constexpr int NC = N / 2;
   static constexpr int CS = NC + 1;                      // complex row stride in LDS
+  static constexpr int RS = 2 * CS;                      // float row stride
   static constexpr int TY = TY_;
+  static constexpr size_t LDS = cmax((int)((size_t)TY * N * 8), (int)((size_t)TY * CS * sizeof(float2)));
+  static constexpr size_t STATIC_LDS = 6144;             // twiddles, bucket tables, sums
+  // blocks of this tile size a CU holds; with two or more, 512 threads each (registers)
+  static constexpr int BPC = (int)((160 * 1024) / (LDS + STATIC_LDS));
   // at least 512 threads: small meshes have dense tiles, and the deposit of their records
   // needs the parallelism even where the z transform alone would use fewer threads
-  static constexpr int NT = cmin(1024, cmax(cmin(512, TY * N), (TY * NC / 8) / 64 * 64));
-  // deposit into 64-bit fixed-point cells when they fit (the FFT rows alias them afterwards)
-  static constexpr bool FIXED = (size_t)TY * N * 8 <= FIXED_LDS_MAX;
-  static constexpr size_t FFT_LDS = (size_t)TY * CS * sizeof(float2);
-  static constexpr size_t LDS = FIXED ? cmax((int)((size_t)TY * N * 8), (int)FFT_LDS) : FFT_LDS;
+  static constexpr int NT0 = cmin(1024, cmax(cmin(512, TY * N), (TY * NC / 8) / 64 * 64));
+  static constexpr int NT = BPC >= 2 ? cmin(NT0, 512) : NT0;
   static constexpr int PER = (TY * N + NT - 1) / NT;     // cells per thread in the conversion
   // waves per SIMD the LDS allows (capped at 4): the register budget is held to it
-  static constexpr int WPE =
-      cmax(1, cmin(4, (int)((160 * 1024) / (LDS + 8192 + (size_t)NT * 8)) * (NT / 64) / 4));
+  static constexpr int WPE = cmax(1, cmin(4, cmax(1, BPC) * (NT / 64) / 4));
 };
 
 // Bucket descriptors of one tile in LDS: first record and exclusive prefix of sizes.
@@ -48,34 +54,36 @@ struct TileBuckets {
   int nbk;
 };
 
-// Block sum of the tile mass: wave butterflies for whole-wave blocks, else the tree.
-template <int NT>
-__device__ __forceinline__ double tile_sum(double v, double* red) {
-  if constexpr (NT % 64 == 0) return block_sum_f64_waves<NT>(v, red);
-  else return block_sum_f64(v, red, NT);
-}
-
+// Deposits are rint(W w S) with S = 2^(30 - e), max |w| < 2^e and corner weights W <= 1, so
+// each is below 2^30 and converts through int32; a cell of R records sums below R 2^30 and a
+// tile's cells below 9 R 2^30, inside int64 for any catalogue of < 2^29 particles.  The
+// quantum 2^(e - 30) must stay this fine: the rounding is white noise on the mesh, and it
+// enters |delta_k|^2 through the cross term with the signal, i.e. as the square root of its
+// power, which the TSC window's deconvolution at the corner of the spectrum exposes
+// (a 2^-18 quantum measured 1e-4 there, 70 times the float32 FFT floor).
 template <int N, int TY>
 __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const float2* __restrict__ twg) {
   using Cf = FusedCfg<N, TY>;
-  constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
+  constexpr int NC = Cf::NC, CS = Cf::CS, RS = Cf::RS, C = Cf::TY, NT = Cf::NT;
   constexpr int U = NT >= 1024 ? 2 : 4;   // records per thread loaded ahead
-  extern __shared__ float2 buf[];   // FFT rows: C rows of CS complex = 2 * CS floats per row
-  __shared__ float2 tw[N];
+  extern __shared__ float2 buf[];   // C x N int64 cells, then C rows of CS complex
+  __shared__ float2 twq[N / 4];
   __shared__ TileBuckets tb[2];
-  __shared__ double red[NT];
+  __shared__ unsigned long long tsum[2];   // exact integer mass of the last two tiles
+  const TwQuarter<N> tw{twq};
   const ScatterArgs a = j.a;
   const TileGeom g = j.g;
   const void* sorted = j.sorted;
   const bool has_w = a.w != nullptr;
   const bool tsc = a.stencil == PK_STENCIL_TSC;
   const bool irregular = a.stencil == PK_STENCIL_CIC_REF;
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(buf);
   float* rows = reinterpret_cast<float*>(buf);
-  unsigned long long* acc = reinterpret_cast<unsigned long long*>(buf);   // [C][N] when FIXED
   const int tid = threadIdx.x;
   const int ntiles = g.ntiles;
   if (bx >= ntiles) return;   // whole block
-  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int i = tid; i < N / 4; i += NT) twq[i] = twg[i];
+  if (tid < 2) tsum[tid] = 0ull;
   const float scale = (float)j.aux[0];
   const double inv_scale = j.aux[1];
   const float c0 = (float)j.stats[3];
@@ -135,27 +143,29 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   };
   prefetch(tb[0]);
   int cur = 0;
+  int t_prev = -1;
   for (int t = bx; t < ntiles; t += nbx) {
     const int xt = t / g.nyt, yt = t % g.nyt;
     const int y0 = yt * C;
     const int tn = t + nbx;
     const TileBuckets& B = tb[cur];
     if (tn < ntiles) issue(tn, lo, hi);
-    if constexpr (Cf::FIXED) {
-      for (int i = tid; i < C * N; i += NT) acc[i] = 0ull;
-    } else {
-      for (int i = tid; i < C * 2 * CS; i += NT) rows[i] = 0.0f;
+    {
+      uint4* z4 = reinterpret_cast<uint4*>(acc);   // C N 8 bytes: whole 16-byte words
+      for (int i = tid; i < C * N / 2; i += NT) z4[i] = make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
-    // fixed point: the record's weight carries the power-of-two scale, so each corner is
-    // (W w) S exactly as before and costs only the round and the int32 conversion
+    // the previous tile's exact mass (every wave added its share before the barrier above)
+    if (tid == 0 && t_prev >= 0) {
+      j.tile_mass[t_prev] = (double)(long long)tsum[cur ^ 1] * inv_scale;
+      tsum[cur ^ 1] = 0ull;
+    }
+    // the record's weight carries the power-of-two scale, so each corner is (W w) S exactly
+    // and costs only the round and the int32 conversion
     auto dep = [&](const float4& r) {
-      for_each_corner_on_x(a, r.x, r.y, r.z, Cf::FIXED ? r.w * scale : r.w, xt, [&](int iy, int iz, float wt) {
+      for_each_corner_on_x(a, r.x, r.y, r.z, r.w * scale, xt, [&](int iy, int iz, float wt) {
         const int ly = iy - y0;
-        if ((unsigned)ly < (unsigned)C) {
-          if constexpr (Cf::FIXED) atomicAdd(&acc[ly * N + iz], to_fixed(wt));
-          else atomicAdd(&rows[ly * 2 * CS + iz], wt);
-        }
+        if ((unsigned)ly < (unsigned)C) atomicAdd(&acc[ly * N + iz], to_fixed(wt));
       });
     };
     const int total = B.pre[B.nbk];
@@ -178,37 +188,32 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       publish(tb[cur ^ 1], lo, hi);
     else
       __syncthreads();
-    // tile mass (FP64) and rho - c into the FFT rows
-    double msum = 0.0;
-    if constexpr (Cf::FIXED) {
+    // rho - c into the FFT rows through registers (the float rows overlap other threads'
+    // cells); the tile's integer mass, exact in 64 bits, with one LDS add per wave
+    {
       float v[Cf::PER];
+      long long isum = 0;
 #pragma unroll
       for (int k = 0; k < Cf::PER; ++k) {
         const int i = tid + k * NT;
         if (i < C * N) {
-          const double q = (double)(long long)acc[i] * inv_scale;
-          msum += q;
-          v[k] = (float)q - c0;
+          const long long q = (long long)acc[i];
+          isum += q;
+          v[k] = (float)((double)q * inv_scale) - c0;
         }
       }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) isum += __shfl_xor(isum, off, 64);
+      if ((tid & 63) == 0) atomicAdd(&tsum[cur], (unsigned long long)isum);
       __syncthreads();   // every cell read before the float rows overwrite them
 #pragma unroll
       for (int k = 0; k < Cf::PER; ++k) {
         const int i = tid + k * NT;
-        if (i < C * N) rows[(i / N) * 2 * CS + (i % N)] = v[k];
-      }
-    } else {
-      for (int i = tid; i < C * N; i += NT) {
-        const int r = i / N, z = i - r * N;
-        float* p = &rows[r * 2 * CS + z];
-        const float v = *p;
-        msum += (double)v;
-        *p = v - c0;
+        if (i < C * N) rows[(i / N) * RS + (i % N)] = v[k];
       }
     }
-    const double m = tile_sum<NT>(msum, red);   // ends with a barrier
-    if (tid == 0) j.tile_mass[t] = m;
     if (tn < ntiles) prefetch(tb[cur ^ 1]);
+    __syncthreads();   // every row converted before the transform reads it
     // z transform of the C rows (pairs of reals as NC complex, then the split); consecutive
     // lanes take consecutive rows, so the butterflies' LDS accesses are free of conflicts
     lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
@@ -220,11 +225,13 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       const float2 fe = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
       const float2 dd = csub(zk, zc);
       const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
-      spec[((size_t)k * N + xt) * N + (size_t)y0 + c] = cadd(fe, cmul(tw[k], fo));
+      spec[((size_t)k * N + xt) * N + (size_t)y0 + c] = cadd(fe, cmul(tw_at(tw, k), fo));
     }
     __syncthreads();   // the split has read the rows before the next tile clears them
+    t_prev = t;
     cur ^= 1;
   }
+  if (tid == 0 && t_prev >= 0) j.tile_mass[t_prev] = (double)(long long)tsum[cur ^ 1] * inv_scale;
 }
 
 template <int N, int TY>
@@ -243,9 +250,7 @@ k_fused_deposit_zpass_jobs(const Job* __restrict__ jobs, const float2* __restric
 
 template <int N, int TY> struct FusedLaunch {
   using Cf = FusedCfg<N, TY>;
-  static bool ok(const TileGeom& g) {
-    return g.ty == TY && g.fixed == (int)Cf::FIXED && Cf::LDS + 16384 <= 160 * 1024;
-  }
+  static bool ok(const TileGeom& g) { return g.ty == TY && Cf::LDS + Cf::STATIC_LDS <= 160 * 1024; }
   template <typename K>
   static int attr(K k) {
     if (Cf::LDS <= 65536) return PK_OK;

@@ -34,6 +34,8 @@ struct TileGeom {
   int chunk;    // particles per counting block (<= 65535: packed 16-bit LDS counters)
   int nblk;     // counting blocks
   int fixed;    // 1: deposit into 64-bit fixed-point LDS cells (ty * n * 8 bytes fit), 0: float
+  int gtab;     // 1: the count and place kernels keep their per-key tables in HBM (the
+                // block's row of the count matrix, global atomics): nkeys too large for LDS
 };
 
 // LDS bytes up to which the deposit accumulates in 64-bit fixed point: gfx950 LDS
@@ -70,7 +72,7 @@ struct Job {
   const double* ksum;          // [nb] sums of sqrt(m) (nullable without PK_FLAG_KMEAN)
   int nb;
   int flags;
-  double* partials;            // [G][nb][3] x-pass block partials
+  double* partials;            // [G][nb][BIN_PART_LEN] x-pass block partials
   const double* dc_part;       // FP64 k = 0 partials of the z pass (nullable)
   int Gz;                      // number of dc_part entries
   const double* scale_stats;   // sums scaled by scale_stats[1]^2 (fused contrast), nullable
@@ -113,6 +115,8 @@ int bin_finalize_jobs(const Job* jobs, int njobs, int max_nb, int G, hipStream_t
 // validates/derives the tile geometry; tile == nullptr -> default rows per tile;
 // tile[0] == 0 -> direct global atomics; otherwise tile[1] = rows per tile
 int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g, bool* direct);
+// most keys whose count / place tables fit LDS (more: HBM tables)
+size_t lds_table_keys();
 // particles per counting block (clamped to the supported range) and the block count
 void set_chunk(TileGeom* g, int64_t np, int64_t chunk);
 size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np);
@@ -130,6 +134,8 @@ struct BatchDims {
   int max_nblk;
   int max_nkeys;
   int max_nb;
+  int lds_keys;   // per-key LDS tables of count / place sized for this many keys (jobs with
+                  // more keys use HBM tables, gtab)
 };
 int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s);
 int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s);

@@ -37,6 +37,7 @@ __global__ void k_copy_or_zero(const float* src, float* dst, int64_t n) {  // sr
 }
 
 constexpr int CNT_NT = 1024;
+constexpr int CNT_UNROLL = 4;
 
 // Batched wrappers: blockIdx.y selects the job; blocks beyond the job's own extent return
 // at once (the whole block, so no barrier is left waiting).
@@ -57,8 +58,14 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
   const TileGeom g = j.g;
   const int tid = threadIdx.x;
   const bool packed = g.chunk <= 65535;
-  const int words = packed ? (g.nkeys + 1) >> 1 : g.nkeys;
-  for (int i = tid; i < words; i += CNT_NT) hist[i] = 0u;
+  unsigned* row = j.mat + (size_t)bx * g.nkeys;
+  if (g.gtab) {   // per-key counters in the block's own row of the count matrix (HBM)
+    for (int k = tid; k < g.nkeys; k += CNT_NT) row[k] = 0u;
+    __threadfence();   // the zeros reach L2, where the atomics below add, before any adds
+  } else {
+    const int words = packed ? (g.nkeys + 1) >> 1 : g.nkeys;
+    for (int i = tid; i < words; i += CNT_NT) hist[i] = 0u;
+  }
   if (tid == 0) wmax_bits = 0u;
   // block 0 clears the estimate's statistics (written from the scans on; no separate fill)
   if (bx == 0 && tid < PK_STATS_LEN) j.stats[tid] = 0.0;
@@ -67,22 +74,33 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
   const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
   double ws = 0.0;
   float wm = 0.0f;
-  for (int64_t i = lo + tid; i < hi; i += CNT_NT) {
-    float px, py, pz, w;
-    load_particle(a, i, px, py, pz, w);
-    const int key = row_tile_key(a, g, px, py, pz);
-    if (key >= 0) {
-      if (packed) atomicAdd(&hist[key >> 1], 1u << (16 * (key & 1)));
-      else atomicAdd(&hist[key], 1u);
-      ws += (double)w;
-      wm = fmaxf(wm, fabsf(w));
+  // CNT_UNROLL particles per thread and iteration: their loads are all in flight before
+  // the first key is computed
+  for (int64_t i0 = lo + tid; i0 < hi; i0 += (int64_t)CNT_NT * CNT_UNROLL) {
+    float px[CNT_UNROLL], py[CNT_UNROLL], pz[CNT_UNROLL], w[CNT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CNT_UNROLL; ++u) {
+      const int64_t i = i0 + (int64_t)u * CNT_NT;
+      if (i < hi) load_particle(a, i, px[u], py[u], pz[u], w[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < CNT_UNROLL; ++u) {
+      if (i0 + (int64_t)u * CNT_NT >= hi) continue;
+      const int key = row_tile_key(a, g, px[u], py[u], pz[u]);
+      if (key >= 0) {
+        if (g.gtab) atomicAdd(&row[key], 1u);
+        else if (packed) atomicAdd(&hist[key >> 1], 1u << (16 * (key & 1)));
+        else atomicAdd(&hist[key], 1u);
+        ws += (double)w[u];
+        wm = fmaxf(wm, fabsf(w[u]));
+      }
     }
   }
   atomicMax(&wmax_bits, __float_as_uint(wm));   // non-negative floats order like their bits
   __syncthreads();
   if (tid == 0) j.wmax[bx] = (double)__uint_as_float(wmax_bits);
-  unsigned* row = j.mat + (size_t)bx * g.nkeys;
-  for (int k = tid; k < g.nkeys; k += CNT_NT) row[k] = packed ? (hist[k >> 1] >> (16 * (k & 1))) & 0xffffu : hist[k];
+  if (!g.gtab)
+    for (int k = tid; k < g.nkeys; k += CNT_NT) row[k] = packed ? (hist[k >> 1] >> (16 * (k & 1))) & 0xffffu : hist[k];
   const double t = block_sum_f64(ws, red, CNT_NT);
   if (tid == 0) j.wsum[bx] = t;
 }
@@ -287,9 +305,11 @@ __device__ __forceinline__ void bucket_place_body(const Job& j, int bx) {
   const TileGeom g = j.g;
   void* sorted = j.sorted;
   const int tid = threadIdx.x;
-  const unsigned* row = j.mat + (size_t)bx * g.nkeys;
-  for (int k = tid; k < g.nkeys; k += CNT_NT) dest[k] = (unsigned)j.key_start[k] + row[k];
-  __syncthreads();
+  unsigned* row = j.mat + (size_t)bx * g.nkeys;   // this block's offset inside every key
+  if (!g.gtab) {
+    for (int k = tid; k < g.nkeys; k += CNT_NT) dest[k] = (unsigned)j.key_start[k] + row[k];
+    __syncthreads();
+  }
   const int64_t lo = (int64_t)bx * g.chunk;
   const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
   const bool has_w = a.w != nullptr;
@@ -306,7 +326,11 @@ __device__ __forceinline__ void bucket_place_body(const Job& j, int bx) {
       const int64_t i = i0 + (int64_t)u * CNT_NT;
       if (i >= hi) continue;
       const int key = row_tile_key(a, g, px[u], py[u], pz[u]);
-      if (key >= 0) store_record(sorted, atomicAdd(&dest[key], 1u), has_w, px[u], py[u], pz[u], w[u]);
+      if (key < 0) continue;
+      // HBM tables: the block's row (offsets after the column scan) advances with global
+      // atomics, each returning this particle's rank
+      const unsigned at = g.gtab ? (unsigned)j.key_start[key] + atomicAdd(&row[key], 1u) : atomicAdd(&dest[key], 1u);
+      store_record(sorted, at, has_w, px[u], py[u], pz[u], w[u]);
     }
   }
 }
@@ -432,6 +456,7 @@ __global__ void __launch_bounds__(MASS_NT) k_mass_finalize_jobs(const Job* __res
 // host side
 // ------------------------------------------------------------------------------------
 constexpr size_t LDS_MAX = 160 * 1024;
+constexpr size_t LDS_TABLE_MAX = LDS_MAX - 16384;   // per-key tables of count / place in LDS
 constexpr int CHUNK_MIN = 4096;          // smallest counting chunk (small catalogues)
 constexpr int CHUNK_MAX = 1 << 22;       // largest (32-bit LDS counters above 65535)
 
@@ -449,6 +474,8 @@ void set_chunk(TileGeom* g, int64_t np, int64_t chunk) {
   if (g->nblk < 1) g->nblk = 1;
 }
 
+size_t lds_table_keys() { return LDS_TABLE_MAX / 4; }
+
 int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g, bool* direct) {
   *direct = false;
   *g = TileGeom{};
@@ -465,23 +492,24 @@ int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g,
     if (ty < 1 || ty > n || n % ty != 0) return set_error(PK_E_INVALID, "rows per tile %d must divide n=%d", ty, n);
     if (ty < 2 && n >= 2) return set_error(PK_E_INVALID, "rows per tile must be >= 2");
   } else {
+    // largest divisor <= 32 whose rows fit the 64-bit fixed-point deposit (n <= 512: 32;
+    // 768 and 1024: 16)
     ty = largest_divisor_le(n, 32);
+    while (ty > 2 && ((size_t)ty * n * 8 > FIXED_LDS_MAX || n % ty != 0)) --ty;
   }
   g->ty = ty;
   g->nyt = n / ty;
   g->ncls = stencil == PK_STENCIL_TSC ? 3 : 2;
   const long ntiles = (long)n * g->nyt;
   const long nkeys = ntiles * g->ncls + 1;
-  const size_t dest_bytes = (size_t)nkeys * 4;   // the place kernel's per-key write positions
   const size_t rows_bytes = (size_t)ty * n * sizeof(float);
-  if (dest_bytes > LDS_MAX - 8192 || rows_bytes > LDS_MAX - 8192) {
-    if (tile) return set_error(PK_E_INVALID, "tile of %d rows x %d needs too much LDS", ty, n);
-    *direct = true;   // very large meshes with TSC: global atomics
-    return PK_OK;
-  }
+  if (rows_bytes > LDS_MAX - 8192) return set_error(PK_E_INVALID, "tile of %d rows x %d needs too much LDS", ty, n);
   g->ntiles = (int)ntiles;
   g->nkeys = (int)nkeys;
   g->fixed = (size_t)ty * n * 8 <= FIXED_LDS_MAX ? 1 : 0;
+  // per-key tables of the count and place kernels: LDS when they fit, else the block's
+  // row of the count matrix in HBM
+  g->gtab = (size_t)nkeys * 4 > LDS_TABLE_MAX ? 1 : 0;
   // one counting / placing block per CU when the catalogue is large (a second, partial
   // round of blocks would cost a whole round), at least CHUNK_MIN particles per block
   int dev = 0;
@@ -517,8 +545,11 @@ int allow_lds(K kernel, size_t bytes) {
   return e;
 }
 
-size_t hist_lds(const TileGeom& g) { return g.chunk <= 65535 ? (size_t)((g.nkeys + 1) / 2) * 4 : (size_t)g.nkeys * 4; }
-size_t dest_lds(int nkeys) { return (size_t)nkeys * 4; }
+size_t hist_lds(const TileGeom& g) {
+  if (g.gtab) return 0;
+  return g.chunk <= 65535 ? (size_t)((g.nkeys + 1) / 2) * 4 : (size_t)g.nkeys * 4;
+}
+size_t dest_lds(const TileGeom& g) { return g.gtab ? 0 : (size_t)g.nkeys * 4; }
 }  // namespace
 
 size_t carve_scatter(Job& j, bool direct, void* base) {
@@ -611,7 +642,7 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
   }
   {
     StageTimer t(ST_SORT_PLACE, s);
-    const size_t lds = dest_lds(g.nkeys);
+    const size_t lds = dest_lds(g);
     int e = allow_lds(k_bucket_place, lds);
     if (e) return e;
     hipLaunchKernelGGL(k_bucket_place, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
@@ -651,7 +682,7 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
   const int nj = d.njobs;
   {
     StageTimer t(ST_SORT_COUNT, s);
-    const size_t lds = (size_t)d.max_nkeys * 4;   // any job may count with 32-bit counters
+    const size_t lds = (size_t)d.lds_keys * 4;   // any job may count with 32-bit counters
     int e = allow_lds(k_bucket_count_jobs, lds);
     if (e) return e;
     hipLaunchKernelGGL(k_bucket_count_jobs, dim3(d.max_nblk, nj), dim3(CNT_NT), lds, s, jobs);
@@ -669,7 +700,7 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
     if (e) return e;
   }
   StageTimer t(ST_SORT_PLACE, s);
-  const size_t lds = dest_lds(d.max_nkeys);
+  const size_t lds = (size_t)d.lds_keys * 4;
   int e = allow_lds(k_bucket_place_jobs, lds);
   if (e) return e;
   hipLaunchKernelGGL(k_bucket_place_jobs, dim3(d.max_nblk, nj), dim3(CNT_NT), lds, s, jobs);

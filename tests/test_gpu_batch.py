@@ -141,6 +141,44 @@ def test_autotune_reports_a_compiled_tile(gpu):
     assert all(v > 0 for v in timings.values())
 
 
-def test_batch_rejects_unsupported_mesh(gpu):
-    with pytest.raises((ValueError, RuntimeError)):
-        batch.BatchEstimator(1024, 2, 1000, 16, device=gpu)
+def test_batched_large_mesh_with_hbm_tables(gpu):
+    # n = 768: 16-row tiles, and every job's count / place tables in HBM (74k-111k keys)
+    cats, _ = make_cats(gpu, [(768, 200000, "CIC", False, "kf"), (768, 100000, "TSC", False, "kf")], seed=9)
+    got = batch.powspec_batched(cats, tile=None, full=True)
+    for c, g in zip(cats, got):
+        same_result(g, plain(c, None))
+
+
+def lognormal_cats(gpu, specs):
+    """Lognormal + RSD mocks (the headline spectrum) from the GPU generator:
+    specs (n, npart, mas, box, seed) -> catalogue dicts and host positions."""
+    from jax_powspec_amd import gauss_field
+    kf = np.geomspace(1e-4, 10.0, 2000)
+    pkf = 2e4 * (kf / 0.05) ** -1.5 * np.exp(-(kf / 0.5) ** 2)
+    cats, host = [], []
+    for n, npart, mas, box, seed in specs:
+        p = gauss_field.lognormal_mock(n, kf, pkf, box, npart / box ** 3, 1.0, seed, growth_rate=0.7, device=gpu)
+        kF = 2 * np.pi / box
+        cats.append(dict(x=p[:, 0], y=p[:, 1], z=p[:, 2], box_size=box, n_bins=n, mas=mas,
+                         k_edges=kF * np.arange(1, n // 2 + 1), shot_noise=box ** 3 / p.shape[0]))
+        host.append(p.cpu().numpy())
+    return cats, host
+
+
+@pytest.mark.timeout(600)
+def test_batched_covariance_mix_against_oracle(gpu):
+    # jobs shaped like the covariance set: n in {128, 192, 256}, 1e5 - 4e6 lognormal + RSD
+    # particles, CIC and TSC, k_F bins (integer edges: ties), two batches by mesh side run
+    # concurrently; every job against the FP64 oracle
+    specs = [(128, 1e5, "CIC", 600.0, 3101), (128, 4e6, "TSC", 1500.0, 3102), (192, 1e6, "CIC", 1000.0, 3103),
+             (192, 3e5, "TSC", 800.0, 3104), (256, 4e6, "CIC", 1400.0, 3105), (256, 2e6, "TSC", 1200.0, 3106)]
+    cats, host = lognormal_cats(gpu, specs)
+    got = batch.powspec_batched(cats, tile=None, full=True)
+    for c, p, g in zip(cats, host, got):
+        k3, pk, nm, ex = g
+        window = 3 if c["mas"] == "TSC" else 2
+        print(f"[covariance mix] n={c['n_bins']} {c['mas']} N_p={p.shape[0]}", flush=True)
+        res = R.estimate(*p.T, None, c["box_size"], c["n_bins"], c["k_edges"], stencil=c["mas"], p=window,
+                         shot_noise=c["shot_noise"])
+        pk_check(pk, nm, ex, res[1] + [c["shot_noise"], 0, 0], res[3], c["box_size"], c["n_bins"],
+                 shot=c["shot_noise"])
