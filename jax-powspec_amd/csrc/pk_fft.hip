@@ -219,61 +219,6 @@ __device__ __forceinline__ int find_bin(int m, const int* __restrict__ thr, cons
   return b;
 }
 
-// Wave64 cross-lane moves (DPP on the VALU, no LDS traffic).  Lanes without a source
-// lane (or in rows the row mask leaves out) read 0.
-template <int CTRL, int ROWS = 0xf>
-__device__ __forceinline__ int dpp_move(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, true);
-}
-template <int CTRL, int ROWS = 0xf>
-__device__ __forceinline__ float dpp_move(float v) {
-  return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v)));
-}
-constexpr int DPP_ROW_SHR = 0x110;   // + shift (1..15), within rows of 16 lanes
-constexpr int DPP_WAVE_SHR1 = 0x138; // lane l reads lane l - 1
-constexpr int DPP_WAVE_SHL1 = 0x130; // lane l reads lane l + 1
-
-// one step of a segmented inclusive scan: (v, f) <- (v + v_src, f_src) where this lane's
-// run has not started yet (f == 0); a lane without a source reads (0, 0) and is unchanged.
-// fc packs an integer count with the flag, fc = 2 cnt + f: while f == 0 its low bit is 0, so
-// fc + fc_src = 2 (cnt + cnt_src) + f_src carries both in one move and one add.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step(float& a, float& b, float& c, int& fc) {
-  const float pa = dpp_move<CTRL, ROWS>(a), pb = dpp_move<CTRL, ROWS>(b), pc = dpp_move<CTRL, ROWS>(c);
-  const int pfc = dpp_move<CTRL, ROWS>(fc);
-  if (!(fc & 1)) {
-    a += pa;
-    b += pb;
-    c += pc;
-    fc += pfc;
-  }
-}
-
-// Sums of the values (and of the integer cnt) over each run of lanes with equal `slot`
-// inside a row of 16 lanes (lane order), left in the run's last lane of the row; returns
-// true in that lane.  A segmented inclusive scan with shifts 1, 2, 4, 8 inside rows of 16:
-// a run that crosses a row boundary is flushed once per row, which costs an extra
-// histogram update but saves the two cross-row steps (row_bcast15 into rows 1 and 3, then
-// row_bcast31 into rows 2 and 3) of a full-wave scan.  Float partial sums of at most 16
-// terms (depth-4 tree: relative error <= 4 ulp of the run's absolute sum); cnt is exact
-// (0 <= cnt, run total < 2^30).  Must be called by all 64 lanes.
-__device__ __forceinline__ bool wave_run_sums(int slot, float& a, float& b, float& c, int& cnt) {
-  const int lane = threadIdx.x & 63;
-  // every cross-lane move runs with all 64 lanes active (a lane the exec mask leaves out
-  // reads as 0 to its neighbours), so none sits behind a short-circuit or a branch
-  const int prev = dpp_move<DPP_WAVE_SHR1>(slot + 1) - 1;   // lane 0 reads -1 (no source)
-  const int f = (lane == 0 || prev != slot) ? 1 : 0;         // this lane starts a run
-  const int next_starts = dpp_move<DPP_WAVE_SHL1>(f);
-  const bool tail = ((lane & 15) == 15) | (next_starts != 0);
-  int fc = 2 * cnt + f;
-  seg_step<DPP_ROW_SHR + 1, 0xf>(a, b, c, fc);
-  seg_step<DPP_ROW_SHR + 2, 0xf>(a, b, c, fc);
-  seg_step<DPP_ROW_SHR + 4, 0xf>(a, b, c, fc);
-  seg_step<DPP_ROW_SHR + 8, 0xf>(a, b, c, fc);
-  cnt = fc >> 1;
-  return tail;
-}
-
 constexpr int HB_LDS = 256;   // bins held in LDS; above this the per-block slice lives in HBM
 constexpr int NACC = BIN_PART_LEN;   // S0, S2, S4 and the binned mode count per block and bin
 
@@ -333,9 +278,21 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;
   }
   __syncthreads();
-  auto bin_of = [&](int m) -> int {
-    if constexpr (LH) return find_bin(m, thr_s, jst_s, nb);
-    else return find_bin(m, thr_g, jst_g, nb);
+  // threshold i (0 <= i <= nb) and the raw bin of m: #{i <= nb : thr[i] <= m} - 1, in
+  // [-1, nb] (-1: below the first edge, nb: at or above the closed last edge)
+  auto thr_at = [&](int i) -> int {
+    if constexpr (LH) return thr_s[i];
+    else return thr_g[i];
+  };
+  auto raw_bin = [&](int m) -> int {
+    int jj = (int)sqrtf((float)m);
+    if (jj * jj > m) --jj;
+    else if ((jj + 1) * (jj + 1) <= m) ++jj;
+    int bb;
+    if constexpr (LH) bb = jst_s[jj];
+    else bb = jst_g[jj];
+    while (bb + 1 <= nb && thr_at(bb + 1) <= m) ++bb;
+    return bb;
   };
   // window factor of array index i (symmetric: index N - i has the same |k|)
   auto wt = [&](int i) -> float {
@@ -347,20 +304,24 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   // [HW yt, HW yt + HW) and its mirror S_b = [N - HW yt - HW, N - HW yt) (HW = C / 2), so one
   // task bins the four entries of (q, +-ky).  Strip columns whose mirror lies in another
   // tile (ky = HW yt and ky = N - HW (yt + 1), the latter -|ky| with |ky| = HW (yt + 1)) are
-  // single-column tasks, which also covers the self-mirrored ky = 0 and ky = N / 2.  Lanes
-  // take LPR = HW + 1 tasks per q row: task s < HW is strip column s (paired with S_b
-  // column HW - s for s >= 1), task HW is S_b column 0.  Without QUAD (C < 16) a task is
-  // one column and the pair (q, N - q).  Every lane walks q in steps of ROWS, so the loop
-  // is wave-uniform (idle lanes carry slot -1) and the wave combines lanes that share a bin
-  // before touching the histogram.
+  // single-column tasks, which also covers the self-mirrored ky = 0 and ky = N / 2.  There
+  // are LPR = HW + 1 tasks per tile: task s < HW is strip column s (paired with S_b column
+  // HW - s for s >= 1), task HW is S_b column 0.  Without QUAD (C < 16) a task is one column
+  // and the pair (q, N - q).
+  // Each lane walks a run of SL consecutive rows q of one task.  Along the run m = q^2 + r2
+  // grows, so its bin only moves forward: one compare per row against the next threshold
+  // tracks it, and the sums of a bin stay in registers until the bin changes or the run
+  // ends, when they are added to the block's histogram.  The lanes of a wave take
+  // consecutive runs of one task, so the bins they add to at the same step differ.
   constexpr bool QUAD = C >= 16;
   constexpr int HW = C / 2;
-  constexpr int LPR = QUAD ? HW + 1 : C;
-  constexpr int ROWS = NT / LPR;
-  static_assert(ROWS >= 1, "at least one q row per pass");
-  constexpr int NIT = (H + 1 + ROWS - 1) / ROWS;
-  const int s = tid % LPR;
-  const int q0 = tid < ROWS * LPR ? tid / LPR : H + 1;   // idle lanes never reach a bin
+  constexpr int LPR = QUAD ? HW + 1 : C;          // tasks per tile
+  constexpr int SEGS = NT / LPR;                  // runs per task
+  static_assert(SEGS >= 1, "at least one run per task");
+  constexpr int SL = (H + 1 + SEGS - 1) / SEGS;   // rows per run
+  const int s = tid / SEGS < LPR ? tid / SEGS : LPR;   // s == LPR: idle lane
+  const int qa = (tid % SEGS) * SL;                    // first row of the run
+  const int qe = s < LPR ? min(qa + SL, H + 1) : qa;   // end of the run (exclusive)
   // global ky column of tile column c in column tile yt
   auto gcol = [](int c, int ytile) -> int {
     if constexpr (QUAD) return c < HW ? HW * ytile + c : N - HW * ytile - 2 * HW + c;
@@ -413,14 +374,14 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     if constexpr (QUAD) {
       kyt = s < HW ? HW * yt + s : HW * (yt + 1);
     } else {
-      const int y = yt * C + s;
+      const int y = yt * C + (s < LPR ? s : 0);
       kyt = y <= H ? y : N - y;
     }
     const int r2 = kyt * kyt + kz2;
-    // bins and window factors do not depend on the transform: the first iteration's table
-    // reads are issued before the tile's load and FFT, each later one an iteration ahead
-    int slot_next = q0 <= H ? bin_of(q0 * q0 + r2) : -1;
-    float wq_next = q0 <= H ? wt(q0) : 0.0f;
+    // the run's first bin and the threshold that ends it do not depend on the transform:
+    // their table reads are issued before the tile's load and FFT
+    int b = qa < qe ? raw_bin(qa * qa + r2) : nb;
+    int thr_next = b < nb ? thr_at(b + 1) : 0x7fffffff;
     const float wyz = wt(kyt) * wt(kz);
     if constexpr (XPF) {
       store_tile(xpre);
@@ -434,63 +395,72 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       if (item + nbx < NTILES) load_tile(tile_of(item + nbx), xpre);
     }
     lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
-    // tile column A of the task, and B with weight fB = 1 for a quad
+    // tile column A of the task, and B for a quad
     const bool quad = QUAD && s >= 1 && s < HW;
-    const int cA = s;
-    const int cB = quad ? 2 * HW - s : s;
-    const float fB = quad ? 1.0f : 0.0f;
-#pragma unroll 1
-    for (int it = 0; it < NIT; ++it) {
-      const int q = q0 + it * ROWS;
-      const int slot = slot_next;
-      const float wqi = wq_next;
-      const int qn = q + ROWS;
-      slot_next = qn <= H ? bin_of(qn * qn + r2) : -1;
-      wq_next = qn <= H ? wt(qn) : 0.0f;
-      // branch-free: idle lanes read rows inside the tile and are zeroed through slot < 0;
-      // the k = 0 mode's power is added by the finalize from the FP64 sum of the mesh
-      // (mu = 0 there, :37)
-      const int m = q * q + r2;
-      const int qr = q < N ? q : 0;
-      const int qm = (q == 0 || q >= N) ? 0 : N - q;
-      const float pair = (q != 0 && q != H) ? 1.0f : 0.0f;
-      const float2 a = buf[qr * PS + cA];
-      const float2 bb = buf[qm * PS + cA];
-      float p2 = (a.x * a.x + a.y * a.y) + pair * (bb.x * bb.x + bb.y * bb.y);
-      if constexpr (QUAD) {
-        const float2 c2 = buf[qr * PS + cB];
-        const float2 d2 = buf[qm * PS + cB];
-        p2 += fB * ((c2.x * c2.x + c2.y * c2.y) + pair * (d2.x * d2.x + d2.y * d2.y));
-      }
-      const bool live = slot >= 0 && m > 0;
-      p2 = live ? p2 * (wqi * wyz) : 0.0f;
-      // stored entries this task bins: (q, ky) and (N - q, ky) when paired, times two for a
-      // quad; the k = 0 mode is counted by the finalize
-      int cnt = live ? (quad ? 2 : 1) * (q != 0 && q != H ? 2 : 1) : 0;
-      // Legendre weights from exact integer numerators (mu^2 = kz^2 / m):
-      //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
-      // so L2 is exactly 0 where 3 kz^2 = m; 1 / m to within 1 ulp (hardware reciprocal)
-      const float rm = __builtin_amdgcn_rcpf((float)(m > 0 ? m : 1));
-      const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
-      const double num4 = (double)kz2 * (35.0 * (double)kz2 - 30.0 * (double)m) + 3.0 * (double)m * (double)m;
-      const float l4 = 0.125f * (float)num4 * rm * rm;
-      float v2 = p2 * l2, v4 = p2 * l4;
-      // runs of equal slot in lane order -> one histogram update per run (its last lane)
-      const bool tail = wave_run_sums(slot, p2, v2, v4, cnt);
-      if (!tail || slot < 0) continue;
+    const int cA = s < LPR ? s : 0;
+    const int cB = quad ? 2 * HW - s : cA;
+    float a0 = 0.0f, a2 = 0.0f, a4 = 0.0f;   // sums of the current bin
+    int cnt = 0, cur = -1;
+    auto flush = [&]() {
+      if (cnt == 0) return;
       if constexpr (LH) {
-        atomicAdd(&hs[slot], (double)p2);
-        atomicAdd(&hs[HB_LDS + slot], (double)v2);
-        atomicAdd(&hs[2 * HB_LDS + slot], (double)v4);
-        atomicAdd(&hc[slot], (unsigned)cnt);
+        atomicAdd(&hs[cur], (double)a0);
+        atomicAdd(&hs[HB_LDS + cur], (double)a2);
+        atomicAdd(&hs[2 * HB_LDS + cur], (double)a4);
+        atomicAdd(&hc[cur], (unsigned)cnt);
       } else {
-        double* g = gpart + (size_t)slot * NACC;
-        atomicAdd(g + 0, (double)p2);
-        atomicAdd(g + 1, (double)v2);
-        atomicAdd(g + 2, (double)v4);
+        double* g = gpart + (size_t)cur * NACC;
+        atomicAdd(g + 0, (double)a0);
+        atomicAdd(g + 1, (double)a2);
+        atomicAdd(g + 2, (double)a4);
         atomicAdd(g + 3, (double)cnt);   // exact: integers below 2^53
       }
+    };
+#pragma unroll 1
+    for (int i = 0; i < SL; ++i) {
+      const int q = qa + i;
+      if (q >= qe) break;
+      const int m = q * q + r2;
+      while (m >= thr_next) {   // the bin moves forward (rarely more than one step)
+        ++b;
+        thr_next = b < nb ? thr_at(b + 1) : 0x7fffffff;
+      }
+      const int slot = b >= 0 && b < nb ? b : -1;
+      if (slot != cur) {
+        flush();
+        cur = slot;
+        a0 = a2 = a4 = 0.0f;
+        cnt = 0;
+      }
+      // the k = 0 mode's power is added by the finalize from the FP64 sum of the mesh
+      // (mu = 0 there, :37), and its count too
+      if (slot < 0 || m == 0) continue;
+      const bool pair = q != 0 && q != H;
+      const int qm = q == 0 ? 0 : N - q;
+      const float2 av = buf[q * PS + cA];
+      const float2 bv = buf[qm * PS + cA];
+      float p2 = (av.x * av.x + av.y * av.y) + (pair ? (bv.x * bv.x + bv.y * bv.y) : 0.0f);
+      if (quad) {
+        const float2 cv = buf[q * PS + cB];
+        const float2 dv = buf[qm * PS + cB];
+        p2 += (cv.x * cv.x + cv.y * cv.y) + (pair ? (dv.x * dv.x + dv.y * dv.y) : 0.0f);
+      }
+      p2 *= wt(q) * wyz;
+      // Legendre weights from integer numerators (mu^2 = kz^2 / m):
+      //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
+      // so L2 is exactly 0 where 3 kz^2 = m; 1 / m to within 1 ulp (hardware reciprocal);
+      // the L4 numerator in float has relative error ~1e-7 of its terms
+      const float rm = __builtin_amdgcn_rcpf((float)m);
+      const float mf = (float)m;
+      const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
+      const float num4 = fmaf((float)kz2, (float)(35 * kz2 - 30 * m), 3.0f * mf * mf);
+      const float l4 = 0.125f * num4 * rm * rm;
+      a0 += p2;
+      a2 = fmaf(p2, l2, a2);
+      a4 = fmaf(p2, l4, a4);
+      cnt += (quad ? 2 : 1) * (pair ? 2 : 1);
     }
+    flush();
     __syncthreads();
   }
   if constexpr (LH) {

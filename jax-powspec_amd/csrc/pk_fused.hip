@@ -61,7 +61,9 @@ struct TileBuckets {
 // enters |delta_k|^2 through the cross term with the signal, i.e. as the square root of its
 // power, which the TSC window's deconvolution at the corner of the spectrum exposes
 // (a 2^-18 quantum measured 1e-4 there, 70 times the float32 FFT floor).
-template <int N, int TY>
+// ST: the stencil fixed at compile time (plain launches), or -1 for the run-time value of
+// each job (batched launches mix stencils)
+template <int N, int TY, int ST>
 __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const float2* __restrict__ twg) {
   using Cf = FusedCfg<N, TY>;
   constexpr int NC = Cf::NC, CS = Cf::CS, RS = Cf::RS, C = Cf::TY, NT = Cf::NT;
@@ -75,8 +77,9 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   const TileGeom g = j.g;
   const void* sorted = j.sorted;
   const bool has_w = a.w != nullptr;
-  const bool tsc = a.stencil == PK_STENCIL_TSC;
-  const bool irregular = a.stencil == PK_STENCIL_CIC_REF;
+  const int stencil = ST >= 0 ? ST : a.stencil;
+  const bool tsc = stencil == PK_STENCIL_TSC;
+  const bool irregular = stencil == PK_STENCIL_CIC_REF;
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(buf);
   float* rows = reinterpret_cast<float*>(buf);
   const int tid = threadIdx.x;
@@ -120,11 +123,12 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
     __syncthreads();
   };
   // record idx (< total) of the tile buckets B; a thread visits increasing idx, so its
-  // bucket cursor b only moves forward
+  // bucket cursor b only moves forward.  The irregular bucket is the last one.
   auto record = [&](const TileBuckets& B, int idx, int& b) -> float4 {
     while (b + 1 < B.nbk && B.pre[b + 1] <= idx) ++b;
     return load_record(sorted, (size_t)B.base[b] + (idx - B.pre[b]), has_w);
   };
+  auto regular = [&](const TileBuckets& B, int idx) -> bool { return !irregular || idx < B.pre[B.nbk - 1]; };
   // Software pipeline over this block's tiles: the next tile's bucket bounds load during the
   // current deposit, and its first U * NT records are loaded into registers before the
   // current tile's FFT, so their latency overlaps the transform and the writes.
@@ -162,17 +166,19 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
     }
     // the record's weight carries the power-of-two scale, so each corner is (W w) S exactly
     // and costs only the round and the int32 conversion
-    auto dep = [&](const float4& r) {
-      for_each_corner_on_x(a, r.x, r.y, r.z, r.w * scale, xt, [&](int iy, int iz, float wt) {
-        const int ly = iy - y0;
-        if ((unsigned)ly < (unsigned)C) atomicAdd(&acc[ly * N + iz], to_fixed(wt));
-      });
+    auto add = [&](int iy, int iz, float wt) {
+      const int ly = iy - y0;
+      if ((unsigned)ly < (unsigned)C) atomicAdd(&acc[ly * N + iz], to_fixed(wt));
+    };
+    auto dep = [&](const float4& r, bool reg) {
+      if (stencil == PK_STENCIL_CIC_REF && reg) ref_corners_on_x_regular(a, r.x, r.y, r.z, r.w * scale, xt, add);
+      else for_each_corner_on_x<ST>(a, r.x, r.y, r.z, r.w * scale, xt, add);
     };
     const int total = B.pre[B.nbk];
     int bc = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (tid + u * NT < total) dep(rp[u]);
+      if (tid + u * NT < total) dep(rp[u], regular(B, tid + u * NT));
     for (int idx0 = tid + U * NT; idx0 < total; idx0 += U * NT) {
       float4 c[U];
 #pragma unroll
@@ -182,7 +188,7 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (idx0 + u * NT < total) dep(c[u]);
+        if (idx0 + u * NT < total) dep(c[u], regular(B, idx0 + u * NT));
     }
     if (tn < ntiles)   // block-uniform; the barrier also closes the deposit
       publish(tb[cur ^ 1], lo, hi);
@@ -234,10 +240,10 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   if (tid == 0 && t_prev >= 0) j.tile_mass[t_prev] = (double)(long long)tsum[cur ^ 1] * inv_scale;
 }
 
-template <int N, int TY>
+template <int N, int TY, int ST>
 __global__ void __launch_bounds__((FusedCfg<N, TY>::NT)) __attribute__((amdgpu_waves_per_eu(FusedCfg<N, TY>::WPE)))
 k_fused_deposit_zpass(Job j, const float2* __restrict__ tw) {
-  fused_body<N, TY>(j, blockIdx.x, gridDim.x, tw);
+  fused_body<N, TY, ST>(j, blockIdx.x, gridDim.x, tw);
 }
 
 template <int N, int TY>
@@ -245,7 +251,7 @@ __global__ void __launch_bounds__((FusedCfg<N, TY>::NT)) __attribute__((amdgpu_w
 k_fused_deposit_zpass_jobs(const Job* __restrict__ jobs, const float2* __restrict__ tw) {
   const Job& j = jobs[blockIdx.y];
   if (!j.active) return;
-  fused_body<N, TY>(j, blockIdx.x, gridDim.x, tw);
+  fused_body<N, TY, -1>(j, blockIdx.x, gridDim.x, tw);
 }
 
 template <int N, int TY> struct FusedLaunch {
@@ -264,7 +270,8 @@ template <int N, int TY> struct FusedLaunch {
     if (!g) {
       int dev = 0, occ = 0;
       (void)hipGetDevice(&dev);
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fused_deposit_zpass<N, TY>, Cf::NT, Cf::LDS) !=
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fused_deposit_zpass<N, TY, PK_STENCIL_CIC>, Cf::NT,
+                                                       Cf::LDS) !=
               hipSuccess ||
           occ < 1)
         occ = 1;
@@ -272,17 +279,23 @@ template <int N, int TY> struct FusedLaunch {
     }
     return g;
   }
-  static int run(const Job& j, const float2* tw, hipStream_t s) {
+  template <int ST>
+  static int run_st(const Job& j, const float2* tw, hipStream_t s) {
     static bool done = false;
     if (!done) {
-      int e = attr(k_fused_deposit_zpass<N, TY>);
+      int e = attr(k_fused_deposit_zpass<N, TY, ST>);
       if (e) return e;
       done = true;
     }
     const int gx = j.g.ntiles < grid() ? j.g.ntiles : grid();
     StageTimer t(ST_DEPOSIT, s);
-    hipLaunchKernelGGL((k_fused_deposit_zpass<N, TY>), dim3(gx), dim3(Cf::NT), Cf::LDS, s, j, tw);
+    hipLaunchKernelGGL((k_fused_deposit_zpass<N, TY, ST>), dim3(gx), dim3(Cf::NT), Cf::LDS, s, j, tw);
     return check_hip(hipGetLastError(), "fused deposit + z pass launch");
+  }
+  static int run(const Job& j, const float2* tw, hipStream_t s) {
+    if (j.a.stencil == PK_STENCIL_CIC_REF) return run_st<PK_STENCIL_CIC_REF>(j, tw, s);
+    if (j.a.stencil == PK_STENCIL_CIC) return run_st<PK_STENCIL_CIC>(j, tw, s);
+    return run_st<PK_STENCIL_TSC>(j, tw, s);
   }
   static int run_jobs(const Job* jobs, int njobs, int ntiles, const float2* tw, hipStream_t s) {
     static bool done = false;

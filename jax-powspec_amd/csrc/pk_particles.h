@@ -159,11 +159,13 @@ __device__ __forceinline__ void for_each_corner(const ScatterArgs& a, float px, 
 // The corners of one particle that lie on the x plane `xt`: f(iy, iz, weight).  Same
 // weights and drop rules as for_each_corner, restricted to one plane, so a tile that owns
 // a single x layer issues 4 (CIC variants) or 9 (TSC) deposits per particle, not 8 or 27.
-template <typename F>
+// ST >= 0 fixes the stencil at compile time (the run-time a.stencil is then ignored).
+template <int ST = -1, typename F>
 __device__ __forceinline__ void for_each_corner_on_x(const ScatterArgs& a, float px, float py, float pz, float w,
                                                      int xt, F&& f) {
   const int n = a.n;
-  if (a.stencil == PK_STENCIL_CIC_REF) {
+  const int stencil = ST >= 0 ? ST : a.stencil;
+  if (stencil == PK_STENCIL_CIC_REF) {
     const RefAxis X = ref_axis(px, n, a.wrap_mode), Y = ref_axis(py, n, a.wrap_mode), Z = ref_axis(pz, n, a.wrap_mode);
     if (X.vb && X.b == xt) {   // src/mas.py:142, :144, :145, :149 (the :149 weight keeps mdy)
       if (Y.vb && Z.vb) f(Y.b, Z.b, X.md * Y.md * Z.md * w);
@@ -177,7 +179,7 @@ __device__ __forceinline__ void for_each_corner_on_x(const ScatterArgs& a, float
       if (Y.vb && Z.vp) f(Y.b, Z.p, X.dd * Y.md * Z.dd * w);
       if (Y.vp && Z.vp) f(Y.p, Z.p, X.dd * Y.dd * Z.dd * w);
     }
-  } else if (a.stencil == PK_STENCIL_CIC) {
+  } else if (stencil == PK_STENCIL_CIC) {
     const float pos[3] = {px, py, pz};
     int c0[3], c1[3];
     float w0[3], w1[3];
@@ -229,6 +231,46 @@ __device__ __forceinline__ void for_each_corner_on_x(const ScatterArgs& a, float
         for (int k = 0; k < 3; ++k) f(c[1][j], c[2][k], axy * wt[2][k]);
       }
     }
+  }
+}
+
+// The corners on plane xt of a reference-stencil particle whose base cell is inside the
+// mesh (every bucket but the irregular one).  Then every index is valid and the four wrap
+// rules give the same +1 neighbour: 0 at the upper edge, i + 1 + n for i + 1 < 0 (negative
+// positions truncate toward 0 and normalise as JAX indexes), else i + 1; only cic_mas with
+// wrap=False also zeroes the fraction at the upper edge (src/mas.py:34).  Same weights and
+// products, left to right, as for_each_corner_on_x (incl. the :149 corner), without its
+// per-corner validity tests.
+template <typename F>
+__device__ __forceinline__ void ref_corners_on_x_regular(const ScatterArgs& a, float px, float py, float pz,
+                                                         float w, int xt, F&& f) {
+  const int n = a.n;
+  const bool scan_false = a.wrap_mode == PK_WRAP_SCAN_FALSE;
+  auto axis = [&](float pos, int& b, int& p, float& md, float& dd) {
+    const int i = (int)pos;   // truncation toward zero
+    const float d = pos - (float)i;
+    md = 1.0f - d;
+    const int ip = i + 1;
+    dd = (scan_false && ip >= n) ? 0.0f : d;
+    b = i < 0 ? i + n : i;
+    p = ip >= n ? 0 : (ip < 0 ? ip + n : ip);
+  };
+  int xb, xp, yb, yp, zb, zp;
+  float xm, xd, ym, yd, zm, zd;
+  axis(px, xb, xp, xm, xd);
+  axis(py, yb, yp, ym, yd);
+  axis(pz, zb, zp, zm, zd);
+  if (xb == xt) {   // src/mas.py:142, :144, :145, :149
+    f(yb, zb, xm * ym * zm * w);
+    f(yp, zb, xm * yd * zm * w);
+    f(yb, zp, xm * ym * zd * w);
+    f(yp, zp, xm * ym * zd * w);
+  }
+  if (xp == xt) {   // src/mas.py:143, :147, :148, :151
+    f(yb, zb, xd * ym * zm * w);
+    f(yp, zb, xd * yd * zm * w);
+    f(yb, zp, xd * ym * zd * w);
+    f(yp, zp, xd * yd * zd * w);
   }
 }
 
@@ -400,10 +442,13 @@ __device__ __forceinline__ void deposit_buckets(const ScatterArgs& a, const Tile
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (idx0 + u * nt >= total) continue;
-      for_each_corner_on_x(a, r[u].x, r[u].y, r[u].z, r[u].w, xt, [&](int iy, int iz, float wt) {
+      auto add = [&](int iy, int iz, float wt) {
         const int ly = iy - y0;
         if ((unsigned)ly < (unsigned)g.ty) f(ly, iz, wt);
-      });
+      };
+      // the tile buckets hold no irregular record (those are added with global atomics)
+      if (a.stencil == PK_STENCIL_CIC_REF) ref_corners_on_x_regular(a, r[u].x, r[u].y, r[u].z, r[u].w, xt, add);
+      else for_each_corner_on_x(a, r[u].x, r[u].y, r[u].z, r[u].w, xt, add);
     }
   }
 }
