@@ -307,16 +307,18 @@ def bench_batched(dev, reps=5):
     groups = {}
     for i, c in enumerate(cats):
         groups.setdefault(c["n_bins"], []).append(i)
-    tiles, tune = {}, {}
+    chosen, tune = {}, {}
     batches = []
     for n, idx in sorted(groups.items()):
         sub = [cats[i] for i in idx]
-        best, timings = batch.autotune_tile(n, sub, reps=3, device=dev)
-        tiles[n] = best[1]
-        tune[n] = {str(k): round(v, 3) for k, v in timings.items()}
+        best, timings = batch.autotune(n, sub, reps=2, device=dev)
+        chosen[n] = {"ty": best["tile"][1], "count_rounds": best["count_rounds"], "ypass_pct": best["ypass_pct"],
+                     "xpass_pct": best["xpass_pct"], "ms": round(best["ms"], 3),
+                     "default_ms": round(timings[(best["tile"][1],) + batch.DEFAULT_KNOBS], 3)}
+        tune[n] = len(timings)
         b = batch.BatchEstimator(n, len(sub), max(int(c["x"].numel()) for c in sub),
-                                 max(len(c["k_edges"]) - 1 for c in sub), tile=best, graph=True, device=dev)
-        b.set(sub)
+                                 max(len(c["k_edges"]) - 1 for c in sub), tile=best["tile"], graph=True, device=dev)
+        b.tune(best["count_rounds"], best["ypass_pct"], best["xpass_pct"]).set(sub)
         batches.append((b, idx))
     bl = [b for b, _ in batches]
     for _ in range(2):   # plain launches, then capture
@@ -351,15 +353,18 @@ def bench_batched(dev, reps=5):
             s_b, s_p = ex_b["sums"], ex_p["sums"]
             scale = s_p.abs().amax(dim=0, keepdim=True).clamp_min(1e-300)
             worst = max(worst, float(((s_b - s_p).abs() / scale).max()))
-    log(f"batched: {batch_s * 1e3:.1f} ms per pass ({plain_s / batch_s:.2f}x), tiles {tiles}")
+    log(f"batched: {batch_s * 1e3:.1f} ms per pass ({plain_s / batch_s:.2f}x), tuned {chosen}")
     return {"mocks": len(cats), "particles": npart, "mesh_sides": {str(n): len(v) for n, v in sorted(groups.items())},
             "plain_ms": round(plain_s * 1e3, 3), "batched_ms": round(batch_s * 1e3, 3),
             "batched_with_refill_ms": round(batch_set_s * 1e3, 3),
             "batched_groups_in_sequence_ms": round(batch_serial_s * 1e3, 3),
             "plain_estimates_per_s": round(len(cats) / plain_s, 1),
             "batched_estimates_per_s": round(len(cats) / batch_s, 1),
-            "speedup": round(plain_s / batch_s, 3), "autotuned_ty": {str(k): v for k, v in tiles.items()},
-            "autotune_ms_per_batch": tune, "max_rel_sum_diff_vs_plain": worst, "counts_equal": counts_equal,
+            "speedup": round(plain_s / batch_s, 3), "autotuned": {str(k): v for k, v in chosen.items()},
+            "autotune_configs_timed": {str(k): v for k, v in tune.items()},
+            "autotune_space": {"ty": "compiled fused variants", "count_rounds": list(batch.COUNT_ROUNDS),
+                               "ypass_pct": list(batch.PASS_PCTS), "xpass_pct": list(batch.PASS_PCTS)},
+            "max_rel_sum_diff_vs_plain": worst, "counts_equal": counts_equal,
             "note": "plain = correlations.estimate() per mock; batched = one graph-replayed launch sequence "
                     "per mesh side over resident slots, the mesh-side groups on concurrent streams; "
                     "inputs resident in HBM"}

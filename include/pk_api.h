@@ -242,6 +242,12 @@ int pk_batch_set_jobs(pk_batch* batch, int njobs, const pk_job* jobs, void* stre
 int pk_batch_run(pk_batch* batch, void* stream);
 /* resolved tile (1, ty, n) of the batch's fused deposit; host int[3] */
 int pk_batch_tile(pk_batch* batch, int* tile);
+/* run-time tuning of a batch (the autotuner's knobs; no reference counterpart):
+ * count_rounds (1..64, default 4) sets the counting chunk so that all jobs' particles take
+ * about that many rounds of one block per CU (takes effect at the next pk_batch_set_jobs);
+ * ypass_pct / xpass_pct (10..400, default 100) scale the y pass's and the x pass's blocks
+ * per job against their resident-capacity defaults.  Drops captured graphs. */
+int pk_batch_set_tuning(pk_batch* batch, int count_rounds, int ypass_pct, int xpass_pct);
 int pk_batch_destroy(pk_batch* batch);
 
 /* --- configuration-space multipoles ----------------------------------------------------

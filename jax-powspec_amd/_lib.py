@@ -81,6 +81,7 @@ SIGNATURES = [
     ("pk_batch_set_jobs", _i, [_vp, _i, _vp, _vp]),
     ("pk_batch_run", _i, [_vp, _vp]),
     ("pk_batch_tile", _i, [_vp, _vp]),
+    ("pk_batch_set_tuning", _i, [_vp, _i, _i, _i]),
     ("pk_batch_destroy", _i, [_vp]),
     ("pk_bispec_workspace_size", _i, [_vp, _i, _psz]),
     ("pk_bispec_mesh", _i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),

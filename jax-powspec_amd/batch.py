@@ -13,8 +13,9 @@ tile: the kernels are the same, only the grid's y dimension selects the job.
     results = powspec_batched(catalogues)          # [(k3D, Pk3D, Nmodes3D), ...] in input order
 
 ``BatchEstimator`` keeps a batch resident (plan, workspace slots, job table, graph) for
-repeated runs over refilled particle buffers; ``autotune_tile`` picks the fused deposit's
-row-bundle height for a mesh side by timing the candidates on a sample batch.
+repeated runs over refilled particle buffers; ``autotune`` picks the fused deposit's
+row-bundle height and the launch knobs (counting chunk, y- and x-pass grids) for a mesh
+side by timing every combination on a sample batch.
 """
 from __future__ import annotations
 
@@ -27,19 +28,27 @@ import torch
 from . import _device, _lib, binning
 from .correlations import _window_p
 
-__all__ = ["BatchEstimator", "powspec_batched", "autotune_tile", "tile_candidates", "run_concurrent", "TUNED_TILES"]
+__all__ = ["BatchEstimator", "powspec_batched", "autotune", "autotune_tile", "tile_candidates", "run_concurrent",
+           "TUNED_TILES", "TUNED_KNOBS"]
 
 # n -> (1, ty, 0) chosen by autotune_tile (per process)
 TUNED_TILES: dict = {}
+# n -> (count_rounds, ypass_pct, xpass_pct) chosen by autotune (per process)
+TUNED_KNOBS: dict = {}
+DEFAULT_KNOBS = (4, 100, 100)
+COUNT_ROUNDS = (1, 2, 4, 8)
+PASS_PCTS = (50, 100)
 
 
 def tile_candidates(n: int):
     """Row-bundle heights with a compiled fused kernel: the default (largest divisor of n
-    that is <= 32) and 16 for n >= 64."""
-    default = n if n <= 32 else (32 if n % 32 == 0 else (24 if n % 24 == 0 else 16))
+    that is <= 32 whose 64-bit fixed-point rows fit 128 KB of LDS: 32 up to n = 512, 16 at
+    768 and 1024) and half of it for n >= 64."""
+    d32 = n if n <= 32 else (32 if n % 32 == 0 else (24 if n % 24 == 0 else 16))
+    default = d32 if d32 * n * 8 <= 131072 else 16
     out = [default]
-    if n >= 64 and n % 16 == 0 and default != 16:
-        out.append(16)
+    if n >= 64 and default % 2 == 0 and n % (default // 2) == 0:
+        out.append(default // 2)
     return [(1, ty, 0) for ty in out]
 
 
@@ -186,6 +195,15 @@ class BatchEstimator:
         self._on_stream(stream, lambda s: _lib.check("pk_batch_run", _lib.load().pk_batch_run(self.handle, s)))
         return self
 
+    def tune(self, count_rounds: int = 4, ypass_pct: int = 100, xpass_pct: int = 100):
+        """Run-time knobs (pk_batch_set_tuning): counting rounds of one block per CU over all
+        particles (applies at the next ``set``) and the y / x passes' blocks per job in
+        percent of their defaults.  Drops captured graphs."""
+        _lib.check("pk_batch_set_tuning", _lib.load().pk_batch_set_tuning(
+            self.handle, int(count_rounds), int(ypass_pct), int(xpass_pct)))
+        self.knobs = (int(count_rounds), int(ypass_pct), int(xpass_pct))
+        return self
+
     def result(self, i: int, full: bool = False):
         j = self.jobs[i]
         nb = j.nb
@@ -224,15 +242,55 @@ def run_concurrent(batches, *, stream=None):
             cur.wait_stream(b._stream)
 
 
-def autotune_tile(n: int, catalogues, *, reps: int = 3, device=None, graph: bool = True):
-    """Time every compiled row-bundle height on ``catalogues`` (same n) and keep the fastest.
+def _time_batch(b, reps):
+    for _ in range(2):   # first run: plain launches; second: graph capture
+        b.run()
+    torch.cuda.synchronize(b.device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        b.run()
+    ev1.record()
+    ev1.synchronize()
+    return ev0.elapsed_time(ev1) / reps
 
-    Returns the chosen (1, ty, 0) and a dict {ty: ms per batch run}; the choice is cached in
-    TUNED_TILES[n] for powspec_batched(tile='auto').
+
+def autotune(n: int, catalogues, *, reps: int = 3, device=None, graph: bool = True):
+    """Search the batched path's tile shape and launch knobs on ``catalogues`` (one mesh
+    side): every compiled row-bundle height, times counting rounds (the sort's chunk size)
+    in COUNT_ROUNDS, times the y- and x-pass grids at PASS_PCTS of their defaults.  Each
+    configuration is timed as graph-replayed batch runs.
+
+    Returns (best, timings): best = {"tile": (1, ty, 0), "count_rounds", "ypass_pct",
+    "xpass_pct", "ms"} and timings = {(ty, rounds, ypct, xpct): ms}; the choice is cached in
+    TUNED_TILES[n] / TUNED_KNOBS[n] for powspec_batched(tile='auto').
     """
     device = _device.require_gpu(device)
-    cats = list(catalogues)
-    jobs = [_Job(c, n, device) for c in cats]
+    jobs = [_Job(c, n, device) for c in catalogues]
+    max_np = max(1, max(j.np for j in jobs))
+    max_nb = max(j.nb for j in jobs)
+    timings = {}
+    best = None
+    for tile in tile_candidates(n):
+        b = BatchEstimator(n, len(jobs), max_np, max_nb, tile=tile, graph=graph, device=device)
+        for rounds in COUNT_ROUNDS:
+            for ypct in PASS_PCTS:
+                for xpct in PASS_PCTS:
+                    b.tune(rounds, ypct, xpct).set(jobs)
+                    ms = _time_batch(b, reps)
+                    timings[(tile[1], rounds, ypct, xpct)] = ms
+                    if best is None or ms < best["ms"]:
+                        best = {"tile": tile, "count_rounds": rounds, "ypass_pct": ypct, "xpass_pct": xpct, "ms": ms}
+        del b
+    TUNED_TILES[n] = best["tile"]
+    TUNED_KNOBS[n] = (best["count_rounds"], best["ypass_pct"], best["xpass_pct"])
+    return best, timings
+
+
+def autotune_tile(n: int, catalogues, *, reps: int = 3, device=None, graph: bool = True):
+    """The row-bundle height alone (default launch knobs): (1, ty, 0) and {ty: ms}."""
+    device = _device.require_gpu(device)
+    jobs = [_Job(c, n, device) for c in catalogues]
     max_np = max(1, max(j.np for j in jobs))
     max_nb = max(j.nb for j in jobs)
     timings = {}
@@ -240,18 +298,8 @@ def autotune_tile(n: int, catalogues, *, reps: int = 3, device=None, graph: bool
     for tile in tile_candidates(n):
         b = BatchEstimator(n, len(jobs), max_np, max_nb, tile=tile, graph=graph, device=device)
         b.set(jobs)
-        for _ in range(2):   # first run: plain launches; second: graph capture
-            b.run()
-        torch.cuda.synchronize(device)
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ev0.record()
-        for _ in range(reps):
-            b.run()
-        ev1.record()
-        ev1.synchronize()
-        ms = ev0.elapsed_time(ev1) / reps
-        timings[tile[1]] = ms
-        if best is None or ms < timings[best[1]]:
+        timings[tile[1]] = _time_batch(b, reps)
+        if best is None or timings[tile[1]] < timings[best[1]]:
             best = tile
         del b
     TUNED_TILES[n] = best
@@ -279,17 +327,18 @@ def powspec_batched(catalogues, *, tile="auto", graph: bool = True, device=None,
     for n, idx in sorted(groups.items()):
         sub = [cats[i] for i in idx]
         t = tile
+        knobs = DEFAULT_KNOBS
         if t == "auto":
-            t = TUNED_TILES.get(n)
-            if t is None:
-                t, _ = autotune_tile(n, sub, device=device, graph=graph)
+            if n not in TUNED_TILES or n not in TUNED_KNOBS:
+                autotune(n, sub, device=device, graph=graph)
+            t, knobs = TUNED_TILES[n], TUNED_KNOBS[n]
         jobs = [_Job(c, n, device) for c in sub]
         cap = max_jobs_per_batch or len(jobs)
         for s0 in range(0, len(jobs), cap):
             part = jobs[s0:s0 + cap]
             b = BatchEstimator(n, len(part), max(1, max(j.np for j in part)), max(j.nb for j in part), tile=t,
                                graph=graph, device=device)
-            b.set(part)
+            b.tune(*knobs).set(part)
             runs.append((b, idx[s0:s0 + cap]))
     run_concurrent([b for b, _ in runs])
     for b, idx in runs:

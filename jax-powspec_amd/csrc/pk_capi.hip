@@ -608,6 +608,11 @@ struct pk_batch {
   int njobs;
   BatchDims dims;
   int gmax;              // x-pass blocks of one job alone (partials rows per slot)
+  // run-time tuning (pk_batch_set_tuning): counting rounds over the GPU, and the y / x
+  // passes' blocks as a percentage of their default (resident-capacity) grids
+  int count_rounds = 4;
+  int ypass_pct = 100;
+  int xpass_pct = 100;
   std::vector<Job> slot;  // scratch pointers of every slot
   struct Graph {
     BatchDims d;
@@ -672,9 +677,12 @@ int batch_launch(pk_batch* b, hipStream_t s) {
   if (e) return e;
   e = mass_finalize_jobs(b->d_jobs, nj, s);
   if (e) return e;
-  e = fft_ypass_jobs(n, b->d_jobs, nj, ypass_jobs_grid(n, nj), tw, s);
+  int gy = (int)((int64_t)ypass_jobs_grid(n, nj) * b->ypass_pct / 100);
+  if (gy < 1) gy = 1;
+  e = fft_ypass_jobs(n, b->d_jobs, nj, gy, tw, s);
   if (e) return e;
-  int gx = bin_jobs_grid(n, b->dims.max_nb, nj);
+  int gx = (int)((int64_t)bin_jobs_grid(n, b->dims.max_nb, nj) * b->xpass_pct / 100);
+  if (gx < 1) gx = 1;
   if (gx > b->gmax) gx = b->gmax;
   e = fft_xpass_bin_jobs(n, b->d_jobs, nj, b->dims.max_nb, gx, tw, s);
   if (e) return e;
@@ -777,7 +785,8 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
   for (int i = 0; i < njobs; ++i) total_np += jobs[i].np > 0 ? jobs[i].np : 0;
   int dev = 0;
   (void)hipGetDevice(&dev);
-  const int64_t batch_chunk = (total_np + 4LL * num_cus(dev) - 1) / (4LL * num_cus(dev));
+  const int64_t rounds = (int64_t)b->count_rounds * num_cus(dev);
+  const int64_t batch_chunk = (total_np + rounds - 1) / rounds;
   for (int i = 0; i < njobs; ++i) {
     const pk_job& q = jobs[i];
     Job j = b->slot[i];
@@ -875,6 +884,24 @@ int pk_batch_run(pk_batch* b, void* stream) {
   }
   b->graphs.push_back(g);
   return check_hip(hipGraphLaunch(g.exec, s), "batch graph launch");
+}
+
+int pk_batch_set_tuning(pk_batch* b, int count_rounds, int ypass_pct, int xpass_pct) {
+  if (!b) return set_error(PK_E_INVALID, "null batch");
+  if (count_rounds < 1 || count_rounds > 64) return set_error(PK_E_INVALID, "count_rounds=%d not in 1..64", count_rounds);
+  if (ypass_pct < 10 || ypass_pct > 400 || xpass_pct < 10 || xpass_pct > 400)
+    return set_error(PK_E_INVALID, "pass grid percentages must be in 10..400");
+  // captured graphs hold the old grids: drop them (the next runs capture again)
+  for (auto& g : b->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  b->graphs.clear();
+  b->seen.clear();
+  b->count_rounds = count_rounds;
+  b->ypass_pct = ypass_pct;
+  b->xpass_pct = xpass_pct;
+  return PK_OK;
 }
 
 int pk_batch_tile(pk_batch* b, int* tile) {

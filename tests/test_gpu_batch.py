@@ -133,6 +133,24 @@ def test_refill_with_fewer_smaller_jobs(gpu):
         b.set(big)
 
 
+def test_autotune_searches_tiles_and_knobs(gpu):
+    cats, _ = make_cats(gpu, [(64, 30000, "CIC", False, "kf"), (64, 30000, "TSC", False, "kf")], seed=10)
+    best, timings = batch.autotune(64, cats, reps=1)
+    tys = {t[1] for t in batch.tile_candidates(64)}
+    assert len(timings) == len(tys) * len(batch.COUNT_ROUNDS) * len(batch.PASS_PCTS) ** 2
+    assert best["tile"][1] in tys and best["ms"] == min(timings.values())
+    # every tuned configuration gives the plain path's results
+    got = batch.powspec_batched(cats, tile="auto", full=True)
+    for c, g in zip(cats, got):
+        same_result(g, plain(c, best["tile"]))
+    b = batch.BatchEstimator(64, 2, 30000, 32, tile=best["tile"], device=gpu)
+    for knobs in [(1, 50, 50), (8, 100, 50), (2, 50, 100)]:
+        b.tune(*knobs).set(cats).run()
+        torch.cuda.synchronize()
+        for i, c in enumerate(cats):
+            same_result(b.result(i, full=True), plain(c, best["tile"]))
+
+
 def test_autotune_reports_a_compiled_tile(gpu):
     cats, _ = make_cats(gpu, [(64, 30000, "CIC", False, "kf"), (64, 30000, "TSC", False, "kf")], seed=8)
     best, timings = batch.autotune_tile(64, cats, reps=2)

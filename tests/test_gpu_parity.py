@@ -1,7 +1,8 @@
 """GPU parity of the HIP path (libpowspec_amd.so, through the C ABI) against the CPU oracle.
 
 Bars: mode counts and bin indices bit-exact (int64 equality); P_l within 1e-5 of the
-same-bin absolute scale A_l,b = (2l+1) sum_b |L_l(mu)| P2 norm / N_b (P_2, P_4 cross zero);
+same-bin absolute scale A_l,b = (2l+1) sum_b |L_l(mu)| P2 norm / N_b (P_2, P_4 cross zero),
+in every bin (beyond k_N too);
 meshes within float32 accumulation error; FFT within float32 rounding.
 """
 import math
@@ -16,7 +17,6 @@ from oracle import ref_fp64 as R
 pytestmark = pytest.mark.gpu
 F32 = np.float32
 PK_TOL = 1e-5
-PK_TOL_CORNER = 1e-4
 
 
 def catalogue(rng, npart, box, clumps=True, edges=True):
@@ -73,16 +73,15 @@ def pk_check(pk, nm, ex, o_pk, o_extra, box, n, shot=0.0, sum_abs=None):
         assert abs(np.sqrt(max(S[slot0, 0], 0)) - np.sqrt(max(So, 0))) <= 1e-12 * (sum_abs or 1.0)
     scale = o_extra["scale"][check]
     err = np.abs(P[check] - o[check]) / np.maximum(scale, 1e-300)
-    # bins beyond the Nyquist sphere (k > k_N = n/2 k_F) see the float32 mesh round-off
-    # amplified by the window deconvolution (up to (pi/2)^(4p) per axis): 1e-4 there
+    # every bin at 1e-5, also beyond the Nyquist sphere (k > k_N = n/2 k_F), where the
+    # deconvolution exposes the float32 mesh and FFT round-off most (reported separately)
     kbin = o_extra["kmean"][check] / (2 * np.pi / box)
     corner = np.isfinite(kbin) & (kbin > n / 2)
-    tol = np.where(corner, PK_TOL_CORNER, PK_TOL)[:, None]
-    worst = float(np.nanmax(err / tol)) if err.size else 0.0
+    worst = float(np.nanmax(err / PK_TOL)) if err.size else 0.0
     below = float(np.nanmax(err[~corner])) if (~corner).any() else 0.0
     above = float(np.nanmax(err[corner])) if corner.any() else 0.0
     print(f"[pk_check n={n}] max eps (same-bin scale): k <= k_N {below:.2e}, k > k_N {above:.2e}", flush=True)
-    assert worst <= 1.0, (worst, np.unravel_index(np.nanargmax(err / tol), err.shape))
+    assert worst <= 1.0, (worst, np.unravel_index(np.nanargmax(err), err.shape))
     # float32 outputs = rounding of (S / N * norm * (2l+1), minus the shot noise on l = 0)
     want32 = (P - np.array([shot, 0.0, 0.0])).astype(F32)
     np.testing.assert_allclose(pk.cpu().numpy()[~empty], want32[~empty], rtol=2e-7, atol=0)

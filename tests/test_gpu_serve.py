@@ -93,3 +93,37 @@ def test_rejects_oversized_requests(gpu):
             srv.submit(reqs[0])
     finally:
         srv.close()
+
+
+@pytest.mark.timeout(300)
+def test_served_results_against_oracle_with_a_1024_request(gpu):
+    # a mix submitted at once (so requests overlap on the device), including a 1024^3 one;
+    # the small-n results against the FP64 oracle at 1e-5 of the same-bin scale, the 1024^3
+    # one against the plain path (the oracle at 1024^3 runs in test_gpu_large_mesh.py)
+    from oracle import ref_fp64 as R
+    specs = [(64, 40000, "CIC", "kf"), (128, 200000, "TSC", "kf"), (1024, 500000, "CIC", "kf"),
+             (32, 10000, "CIC_REF", "fundamental"), (96, 80000, "TSC", "fundamental")]
+    reqs = requests(gpu, specs, seed=7)
+    srv = PkServer(device=gpu, max_batch=4, max_particles=600_000)
+    try:
+        got = [f.result(timeout=120) for f in [srv.submit(r) for r in reqs]]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["errors"] == 0
+    for r, g in zip(reqs, got):
+        n = r["n_bins"]
+        print(f"[served] n={n} {r['mas']} N_p={r['x'].numel()}", flush=True)
+        if n >= 1024:
+            same(g, plain(r))
+            continue
+        pos = torch.stack([r["x"], r["y"], r["z"]], 1).cpu().numpy()
+        window = 3 if r["mas"] == "TSC" else 2
+        res = R.estimate(*pos.T, None, r["box_size"], n, r["k_edges"], stencil=r["mas"], p=window,
+                         shot_noise=r["shot_noise"])
+        opk, oex = np.asarray(res[1], np.float64), res[3]
+        np.testing.assert_array_equal(g[2], oex["counts"].astype(np.float32))
+        ok = oex["counts"] > 0
+        assert np.all(np.isnan(g[1][~ok]))
+        err = np.abs(g[1][ok].astype(np.float64) - opk[ok]) / oex["scale"][ok]
+        assert np.nanmax(err) <= 1e-5, np.nanmax(err)
