@@ -141,6 +141,19 @@ template <int M> __device__ __forceinline__ float2 tw_at(TwQuarter<M> t, int q) 
   return (b & 2) ? make_float2(-r.x, -r.y) : r;
 }
 
+// first half q < M / 2 (in LDS), with w^(q + M/2) = -w^q: half the table of the full
+// one, and a negation instead of the quarter table's rotation logic
+template <int M> struct TwHalf {
+  const float2* p;
+};
+template <int M> __device__ __forceinline__ float2 tw_at(TwHalf<M> t, int q) {
+  constexpr int H = M / 2;
+  static_assert(M % 2 == 0, "half tables need M even");
+  const bool b = q >= H;
+  const float2 v = t.p[b ? q - H : q];
+  return b ? make_float2(-v.x, -v.y) : v;
+}
+
 // ------------------------------------------------------------------------------------
 // One Stockham stage over C independent length-N transforms held in LDS.
 // Element (point p, column c) lives at buf[p * PS + c * CS].  The stage reads radix-R

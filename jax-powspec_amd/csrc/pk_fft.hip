@@ -121,11 +121,11 @@ __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const floa
   constexpr int C = Cf::CY, NT = Cf::NTY, CS = N + (SK > 0 ? (N >> SK) : 0) + 1;
   const int NTILES = ntiles < 0 ? (N / 2 + 1) * N / C : ntiles;
   __shared__ float2 buf[C * CS];
-  __shared__ float2 twq[N / 4];
-  const TwQuarter<N> tw{twq};
+  __shared__ float2 twh[N / 2];
+  const TwHalf<N> tw{twh};
   constexpr int PER = (C * N + NT - 1) / NT;
   const int tid = threadIdx.x;
-  for (int i = tid; i < N / 4; i += NT) twq[i] = twg[i];
+  for (int i = tid; i < N / 2; i += NT) twh[i] = twg[i];
   auto at = [](int p, int c) { return c * CS + p + (SK > 0 ? (p >> SK) : 0); };
   // the next tile's rows are loaded into registers while this tile is transformed
   float2 pre[PER];
@@ -234,7 +234,7 @@ template <int N> struct XBinLds {
   static constexpr int JLEN = isqrt_c(3 * (N / 2) * (N / 2)) + 2;   // isqrt-start table entries
   static constexpr size_t HIST =
       (size_t)(HB_LDS + 1) * 4 + 3 * HB_LDS * 8 + HB_LDS * 4 + (size_t)(N / 2 + 1) * 4 + (size_t)JLEN * 2 +
-      (size_t)(N / 4) * 8;
+      (size_t)(N / 2) * 8;
   // LDS-limited waves per SIMD (the histogram in HBM: no limit is imposed)
   template <bool LH> static constexpr int wpe() {
     return LH ? cmax(1, cmin(4, (int)((160 * 1024) / (TILE + HIST)) * (FFTCfg<N>::NTX / 64) / 4)) : 1;
@@ -254,7 +254,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   constexpr int NTILES = (N / 2 + 1) * YT;
   constexpr int H = N / 2;
   // LDS: the tile, the S_0,2,4 histogram, the thresholds, the window factors of |k| <= N/2,
-  // the isqrt-start table and the first quarter of the twiddles (<= 80 KB at N = 512 so two
+  // the isqrt-start table and the first half of the twiddles (<= 80 KB at N = 512 so two
   // blocks share a CU)
   constexpr int JLEN = XBinLds<N>::JLEN;
   __shared__ float2 buf[N * PS];
@@ -263,10 +263,10 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   __shared__ unsigned hc[LH ? HB_LDS : 1];   // stored modes binned per bin
   __shared__ float wt_s[LH ? H + 1 : 1];
   __shared__ short jst_s[LH ? JLEN : 1];   // bins < HB_LDS
-  __shared__ float2 twq[N / 4];
-  const TwQuarter<N> tw{twq};
+  __shared__ float2 twh[N / 2];
+  const TwHalf<N> tw{twh};
   const int tid = threadIdx.x;
-  for (int i = tid; i < N / 4; i += NT) twq[i] = twg[i];
+  for (int i = tid; i < N / 2; i += NT) twh[i] = twg[i];
   double* gpart = j.partials + (size_t)bx * nb * NACC;
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];

@@ -26,16 +26,18 @@ template <int N> struct FusedTY {
   static constexpr int ALT = (N >= 64 && DEF % 2 == 0 && N % (DEF / 2) == 0) ? DEF / 2 : DEF;
 };
 
-// The tile's cells are 64-bit fixed-point integers [TY][N]; after the deposit they are
-// converted to float rows [TY][N + 2] (the z transform's complex rows of N/2 + 1) in the
-// same LDS.
+// The tile's cells are 64-bit fixed-point integers stored z-major, [N][TY] (cell (row, z) at
+// z TY + row), so the lanes of the z transform's first stage, which take consecutive rows,
+// read consecutive cells.  That stage converts them to float (rho - c) in registers and
+// writes its outputs as the complex rows [TY][N/2 + 1] that the later stages use, in the
+// same LDS (all of its reads finish before its writes).
 template <int N, int TY_> struct FusedCfg {
   static constexpr int NC = N / 2;
   static constexpr int CS = NC + 1;                      // complex row stride in LDS
   static constexpr int RS = 2 * CS;                      // float row stride
   static constexpr int TY = TY_;
   static constexpr size_t LDS = cmax((int)((size_t)TY * N * 8), (int)((size_t)TY * CS * sizeof(float2)));
-  static constexpr size_t STATIC_LDS = 6144;             // twiddles, bucket tables, sums
+  static constexpr size_t STATIC_LDS = (size_t)N * 8 + 2048;   // twiddles, bucket tables, sums
   // blocks of this tile size a CU holds; with two or more, 512 threads each (registers)
   static constexpr int BPC = (int)((160 * 1024) / (LDS + STATIC_LDS));
   // at least 512 threads: small meshes have dense tiles, and the deposit of their records
@@ -66,13 +68,12 @@ struct TileBuckets {
 template <int N, int TY, int ST>
 __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const float2* __restrict__ twg) {
   using Cf = FusedCfg<N, TY>;
-  constexpr int NC = Cf::NC, CS = Cf::CS, RS = Cf::RS, C = Cf::TY, NT = Cf::NT;
+  constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
   constexpr int U = NT >= 1024 ? 2 : 4;   // records per thread loaded ahead
-  extern __shared__ float2 buf[];   // C x N int64 cells, then C rows of CS complex
-  __shared__ float2 twq[N / 4];
+  extern __shared__ float2 buf[];   // N x C int64 cells (z-major), then C rows of CS complex
+  __shared__ float2 tw[N];   // full table: the split and the stages read one entry each
   __shared__ TileBuckets tb[2];
   __shared__ unsigned long long tsum[2];   // exact integer mass of the last two tiles
-  const TwQuarter<N> tw{twq};
   const ScatterArgs a = j.a;
   const TileGeom g = j.g;
   const void* sorted = j.sorted;
@@ -81,11 +82,10 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   const bool tsc = stencil == PK_STENCIL_TSC;
   const bool irregular = stencil == PK_STENCIL_CIC_REF;
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(buf);
-  float* rows = reinterpret_cast<float*>(buf);
   const int tid = threadIdx.x;
   const int ntiles = g.ntiles;
   if (bx >= ntiles) return;   // whole block
-  for (int i = tid; i < N / 4; i += NT) twq[i] = twg[i];
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   if (tid < 2) tsum[tid] = 0ull;
   const float scale = (float)j.aux[0];
   const double inv_scale = j.aux[1];
@@ -168,7 +168,7 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
     // and costs only the round and the int32 conversion
     auto add = [&](int iy, int iz, float wt) {
       const int ly = iy - y0;
-      if ((unsigned)ly < (unsigned)C) atomicAdd(&acc[ly * N + iz], to_fixed(wt));
+      if ((unsigned)ly < (unsigned)C) atomicAdd(&acc[iz * C + ly], to_fixed(wt));
     };
     auto dep = [&](const float4& r, bool reg) {
       if (stencil == PK_STENCIL_CIC_REF && reg) ref_corners_on_x_regular(a, r.x, r.y, r.z, r.w * scale, xt, add);
@@ -194,35 +194,50 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       publish(tb[cur ^ 1], lo, hi);
     else
       __syncthreads();
-    // rho - c into the FFT rows through registers (the float rows overlap other threads'
-    // cells); the tile's integer mass, exact in 64 bits, with one LDS add per wave
+    // The z transform (pairs of reals as NC complex, then the split).  Its first radix-R0
+    // stage reads the integer cells, converts rho - c in registers and sums the tile's
+    // integer mass (exact in 64 bits, one LDS add per wave); consecutive lanes take
+    // consecutive rows, so the butterflies' LDS accesses are free of conflicts.
     {
-      float v[Cf::PER];
+      constexpr int R0 = radix_at(NC, 0);
+      constexpr int B0 = NC / R0;
+      constexpr int TASKS = B0 * C;
+      constexpr int PER0 = (TASKS + NT - 1) / NT;
+      float2 v[PER0][R0];
       long long isum = 0;
 #pragma unroll
-      for (int k = 0; k < Cf::PER; ++k) {
-        const int i = tid + k * NT;
-        if (i < C * N) {
-          const long long q = (long long)acc[i];
-          isum += q;
-          v[k] = (float)((double)q * inv_scale) - c0;
+      for (int u = 0; u < PER0; ++u) {
+        const int task = tid + u * NT;
+        if (TASKS % NT == 0 || task < TASKS) {
+          const int jj = task / C, c = task % C;
+#pragma unroll
+          for (int r = 0; r < R0; ++r) {
+            const int p = jj + r * B0;   // complex point p = cells z = 2p, 2p + 1
+            const long long q0 = (long long)acc[(2 * p) * C + c];
+            const long long q1 = (long long)acc[(2 * p + 1) * C + c];
+            isum += q0 + q1;
+            v[u][r] = make_float2((float)((double)q0 * inv_scale) - c0, (float)((double)q1 * inv_scale) - c0);
+          }
         }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) isum += __shfl_xor(isum, off, 64);
       if ((tid & 63) == 0) atomicAdd(&tsum[cur], (unsigned long long)isum);
-      __syncthreads();   // every cell read before the float rows overwrite them
+      if (tn < ntiles) prefetch(tb[cur ^ 1]);
+      __syncthreads();   // every cell read before the complex rows overwrite them
 #pragma unroll
-      for (int k = 0; k < Cf::PER; ++k) {
-        const int i = tid + k * NT;
-        if (i < C * N) rows[(i / N) * RS + (i % N)] = v[k];
+      for (int u = 0; u < PER0; ++u) {
+        const int task = tid + u * NT;
+        if (TASKS % NT == 0 || task < TASKS) {
+          const int jj = task / C, c = task % C;
+          DFT<R0>::run(v[u]);
+#pragma unroll
+          for (int r = 0; r < R0; ++r) buf[(jj * R0 + r) + c * CS] = v[u][r];
+        }
       }
+      __syncthreads();
+      FFTStages<NC, 1, R0, C, 1, CS, NT, true, 2, 0>::run(buf, tw);
     }
-    if (tn < ntiles) prefetch(tb[cur ^ 1]);
-    __syncthreads();   // every row converted before the transform reads it
-    // z transform of the C rows (pairs of reals as NC complex, then the split); consecutive
-    // lanes take consecutive rows, so the butterflies' LDS accesses are free of conflicts
-    lds_fft<NC, C, 1, CS, NT, true, 2>(buf, tw);
     for (int i = tid; i < (NC + 1) * C; i += NT) {
       const int c = i % C, k = i / C;
       const float2 zk = buf[c * CS + (k == NC ? 0 : k)];
@@ -231,7 +246,7 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       const float2 fe = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
       const float2 dd = csub(zk, zc);
       const float2 fo = make_float2(0.5f * dd.y, -0.5f * dd.x);
-      spec[((size_t)k * N + xt) * N + (size_t)y0 + c] = cadd(fe, cmul(tw_at(tw, k), fo));
+      spec[((size_t)k * N + xt) * N + (size_t)y0 + c] = cadd(fe, cmul(tw[k], fo));
     }
     __syncthreads();   // the split has read the rows before the next tile clears them
     t_prev = t;
