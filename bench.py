@@ -86,7 +86,8 @@ def algorithmic_bytes(n, npart, fused, rec=12):
     half = 8 * n * n * (n // 2 + 1)
     ab = {
         "sort_count": 12 * npart,                  # read positions
-        "sort_place": 12 * npart + rec * npart,    # read positions, write sorted records
+        "sort_place": 12 * npart + rec * npart,    # read positions, write plane-grouped records
+        "sort_fine": 2 * rec * npart,              # read them, write the key-sorted records
         "ypass": 2 * half,                         # in place
         "xpass_bin": half,                         # read; only nb-sized sums leave
     }
@@ -100,7 +101,7 @@ def algorithmic_bytes(n, npart, fused, rec=12):
 
 # stage -> kernel-name fragment in rocprofv3 output
 STAGE_KERNELS = {
-    "sort_count": "k_bucket_count", "sort_place": "k_bucket_place", "ypass": "k_ypass<",
+    "sort_count": "k_bucket_count", "sort_place": "k_bucket_place", "sort_fine": "k_fine_place", "ypass": "k_ypass<",
     "xpass_bin": "k_xpass_bin<", "zpass": "k_zpass<",
 }
 

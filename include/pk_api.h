@@ -353,7 +353,7 @@ int pk_populate_place(const float* field, const float* psi_z, double growth_rate
  * pk_timing_read synchronises those events, returns total ms and launch counts per stage
  * (arrays of PK_NUM_STAGES) and clears the record.
  */
-#define PK_NUM_STAGES 14
+#define PK_NUM_STAGES 15
 int pk_timing_enable(int on);
 int pk_timing_read(double* ms, int64_t* launches, int nstages);
 const char* pk_stage_name(int stage);

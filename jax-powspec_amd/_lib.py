@@ -23,7 +23,7 @@ WRAP_VEC_TRUE, WRAP_VEC_FALSE, WRAP_SCAN_TRUE, WRAP_SCAN_FALSE = 0, 1, 2, 3
 FLAG_ZERO_DC = 1
 FLAG_KMEAN = 2
 STATS_LEN = 4
-NUM_STAGES = 14
+NUM_STAGES = 15
 NOALLOC_SCRATCH_BYTES = 16 << 20
 
 # (name, restype, argtypes) for every symbol declared in include/pk_api.h

@@ -57,7 +57,7 @@ thread_local hipEvent_t g_open[PK_NUM_STAGES] = {};
 const char* g_stage_names[PK_NUM_STAGES] = {"sort_count", "scan",   "sort_place",  "deposit",
                                             "atomic",     "mass",   "zpass",       "ypass",
                                             "xpass_bin",  "finalize", "xpass_plain", "fill",
-                                            "adjoint_fft", "gather"};
+                                            "adjoint_fft", "gather",   "sort_fine"};
 }  // namespace
 
 void timing_suspend(bool on);
@@ -634,7 +634,7 @@ bool timing_suspended() { return t_capturing; }
 namespace {
 bool same_dims(const BatchDims& a, const BatchDims& b) {
   return a.njobs == b.njobs && a.max_nblk == b.max_nblk && a.max_nkeys == b.max_nkeys && a.max_nb == b.max_nb &&
-         a.lds_keys == b.lds_keys;
+         a.lds_keys == b.lds_keys && a.n == b.n && a.max_fblk == b.max_fblk;
 }
 
 int batch_geometry(int n, int64_t max_np, const int* tile, TileGeom* cap) {
@@ -756,7 +756,7 @@ int pk_batch_create(pk_plan* p, int max_jobs, int64_t max_np, int max_nb, const 
   }
   b->cur = 0;
   b->njobs = 0;
-  b->dims = BatchDims{0, 0, 0, 0};
+  b->dims = BatchDims{0, 0, 0, 0, 0, p->n, 1};
   *out = b;
   return PK_OK;
 }
@@ -774,7 +774,7 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     b->pending[k] = false;
   }
   Job* h = b->h_jobs[k];
-  BatchDims d{njobs, 1, 1, 1, 0};
+  BatchDims d{njobs, 1, 1, 1, 0, b->n, 1};
   // capacity grids: one LDS table size for every refill (the capacity geometry's keys if
   // they fit, else none), so captured launches repeat; jobs with more keys use HBM tables
   const int cap_lds_keys = (size_t)b->cap.nkeys <= lds_table_keys() ? b->cap.nkeys : 0;
@@ -818,6 +818,7 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     h[i] = j;
     if (j.g.nblk > d.max_nblk) d.max_nblk = j.g.nblk;
     if (j.g.nkeys > d.max_nkeys) d.max_nkeys = j.g.nkeys;
+    if (fine_place_blocks(q.np) > d.max_fblk) d.max_fblk = fine_place_blocks(q.np);
     if (q.nb > d.max_nb) d.max_nb = q.nb;
     if (b->flags & PK_BATCH_CAPACITY_GRIDS) {
       if (j.g.nkeys > cap_lds_keys) h[i].g.gtab = 1;
@@ -829,6 +830,7 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     d.max_nblk = b->cap.nblk;
     d.max_nkeys = b->cap.nkeys;
     d.lds_keys = cap_lds_keys;
+    d.max_fblk = fine_place_blocks(b->max_np);
   }
   hipStream_t s = as_stream(stream);
   if (njobs > 0) {

@@ -16,7 +16,7 @@ int check_hip(hipError_t e, const char* what);
 
 enum Stage {
   ST_SORT_COUNT = 0, ST_SCAN, ST_SORT_PLACE, ST_DEPOSIT, ST_ATOMIC, ST_MASS,
-  ST_ZPASS, ST_YPASS, ST_XPASS_BIN, ST_FINALIZE, ST_XPASS_PLAIN, ST_FILL, ST_ADJOINT_FFT, ST_GATHER
+  ST_ZPASS, ST_YPASS, ST_XPASS_BIN, ST_FINALIZE, ST_XPASS_PLAIN, ST_FILL, ST_ADJOINT_FFT, ST_GATHER, ST_SORT_FINE
 };
 void timing_begin(int stage, hipStream_t s);
 void timing_end(int stage, hipStream_t s);

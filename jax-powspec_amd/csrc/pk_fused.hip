@@ -69,7 +69,7 @@ template <int N, int TY, int ST>
 __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const float2* __restrict__ twg) {
   using Cf = FusedCfg<N, TY>;
   constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
-  constexpr int U = NT >= 1024 ? 2 : 4;   // records per thread loaded ahead
+  constexpr int U = 2;   // records per thread loaded ahead
   extern __shared__ float2 buf[];   // N x C int64 cells (z-major), then C rows of CS complex
   __shared__ float2 tw[N];   // full table: the split and the stages read one entry each
   __shared__ TileBuckets tb[2];
@@ -122,11 +122,20 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
     }
     __syncthreads();
   };
-  // record idx (< total) of the tile buckets B; a thread visits increasing idx, so its
-  // bucket cursor b only moves forward.  The irregular bucket is the last one.
-  auto record = [&](const TileBuckets& B, int idx, int& b) -> float4 {
-    while (b + 1 < B.nbk && B.pre[b + 1] <= idx) ++b;
-    return load_record(sorted, (size_t)B.base[b] + (idx - B.pre[b]), has_w);
+  // record idx of the tile buckets B (clamped to the last one, so the loads of a thread's
+  // records need no branch and are all in flight together; callers test idx < total at the
+  // deposit): a fixed four-step search for the bucket holding idx.  The irregular bucket is
+  // the last one.
+  static_assert(MAX_TILE_BUCKETS <= 16, "four search steps");
+  auto record = [&](const TileBuckets& B, int idx) -> float4 {
+    const int i = min(idx, B.pre[B.nbk] - 1);
+    int b = 0;
+#pragma unroll
+    for (int step = 8; step >= 1; step >>= 1) {
+      const int p = B.pre[b + step];   // b + step <= 15: inside pre[] even beyond nbk
+      b = (b + step < B.nbk && p <= i) ? b + step : b;
+    }
+    return load_record_raw(sorted, (size_t)B.base[b] + (i - B.pre[b]), has_w);
   };
   auto regular = [&](const TileBuckets& B, int idx) -> bool { return !irregular || idx < B.pre[B.nbk - 1]; };
   // Software pipeline over this block's tiles: the next tile's bucket bounds load during the
@@ -137,12 +146,9 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   publish(tb[0], lo, hi);
   float4 rp[U];
   auto prefetch = [&](const TileBuckets& B) {
-    const int total = B.pre[B.nbk];
-    int b = 0;
+    if (B.pre[B.nbk] > 0) {   // block-uniform
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int idx = tid + u * NT;
-      if (idx < total) rp[u] = record(B, idx, b);
+      for (int u = 0; u < U; ++u) rp[u] = record(B, tid + u * NT);
     }
   };
   prefetch(tb[0]);
@@ -171,21 +177,19 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       if ((unsigned)ly < (unsigned)C) atomicAdd(&acc[iz * C + ly], to_fixed(wt));
     };
     auto dep = [&](const float4& r, bool reg) {
-      if (stencil == PK_STENCIL_CIC_REF && reg) ref_corners_on_x_regular(a, r.x, r.y, r.z, r.w * scale, xt, add);
-      else for_each_corner_on_x<ST>(a, r.x, r.y, r.z, r.w * scale, xt, add);
+      const float ws = record_weight(r, has_w) * scale;
+      if (stencil == PK_STENCIL_CIC_REF && reg) ref_corners_on_x_regular(a, r.x, r.y, r.z, ws, xt, add);
+      else for_each_corner_on_x<ST>(a, r.x, r.y, r.z, ws, xt, add);
     };
     const int total = B.pre[B.nbk];
-    int bc = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (tid + u * NT < total) dep(rp[u], regular(B, tid + u * NT));
     for (int idx0 = tid + U * NT; idx0 < total; idx0 += U * NT) {
       float4 c[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int idx = idx0 + u * NT;
-        if (idx < total) c[u] = record(B, idx, bc);
-      }
+      for (int u = 0; u < U; ++u) c[u] = record(B, idx0 + u * NT);
+      __builtin_amdgcn_sched_barrier(0);   // every load issued before the first deposit
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (idx0 + u * NT < total) dep(c[u], regular(B, idx0 + u * NT));

@@ -61,6 +61,8 @@ struct Job {
   double* wmax;          // [nblk] max |w|
   double* aux;           // [AUX_LEN] fixed-point scale
   void* sorted;          // records
+  void* staged;          // records grouped by x plane only (the first placing pass)
+  unsigned* koff;        // [nkeys] records placed so far in each key (second pass)
   double* tile_mass;     // [ntiles]
   double* stats;         // [PK_STATS_LEN]
   // spectrum and binning
@@ -115,8 +117,10 @@ int bin_finalize_jobs(const Job* jobs, int njobs, int max_nb, int G, hipStream_t
 // validates/derives the tile geometry; tile == nullptr -> default rows per tile;
 // tile[0] == 0 -> direct global atomics; otherwise tile[1] = rows per tile
 int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g, bool* direct);
-// most keys whose count / place tables fit LDS (more: HBM tables)
+// most keys whose count tables fit LDS (more: HBM tables)
 size_t lds_table_keys();
+// blocks of the second placing pass for np particles
+int fine_place_blocks(int64_t np);
 // particles per counting block (clamped to the supported range) and the block count
 void set_chunk(TileGeom* g, int64_t np, int64_t chunk);
 size_t scatter_workspace_bytes(const TileGeom& g, bool direct, int64_t np);
@@ -134,8 +138,10 @@ struct BatchDims {
   int max_nblk;
   int max_nkeys;
   int max_nb;
-  int lds_keys;   // per-key LDS tables of count / place sized for this many keys (jobs with
-                  // more keys use HBM tables, gtab)
+  int lds_keys;   // per-key LDS tables of count sized for this many keys (jobs with more
+                  // keys use HBM tables, gtab)
+  int n;          // mesh side of every job
+  int max_fblk;   // blocks of the second placing pass of the largest job
 };
 int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s);
 int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s);

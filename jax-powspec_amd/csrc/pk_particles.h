@@ -21,21 +21,32 @@ __device__ __forceinline__ int floor_mod(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
+// AOS / W: the input layout (x, y, z interleaved) and the presence of weights, fixed at
+// compile time (0 / 1) or read from a (-1).  Fixed values leave no branch between the loads
+// of a thread's particles, so they are all in flight together; the position arithmetic
+// follows the loads.
+template <int AOS = -1, int W = -1>
 __device__ __forceinline__ void load_particle(const ScatterArgs& a, int64_t i, float& px, float& py, float& pz,
                                               float& w) {
-  if (a.aos) {   // one 12-byte load per particle
+  const bool aos = AOS >= 0 ? AOS == 1 : a.aos != 0;
+  const bool hw = W >= 0 ? W == 1 : a.w != nullptr;
+  float x, y, z;
+  if (aos) {   // one 12-byte load per particle
     typedef float f3 __attribute__((ext_vector_type(3)));
     f3 v;
     __builtin_memcpy(&v, a.x + 3 * i, sizeof(float) * 3);
-    px = (v.x - a.xmin) * a.inv;
-    py = (v.y - a.ymin) * a.inv;
-    pz = (v.z - a.zmin) * a.inv;
+    x = v.x;
+    y = v.y;
+    z = v.z;
   } else {
-    px = (a.x[i * a.sx] - a.xmin) * a.inv;
-    py = (a.y[i * a.sy] - a.ymin) * a.inv;
-    pz = (a.z[i * a.sz] - a.zmin) * a.inv;
+    x = a.x[i * a.sx];
+    y = a.y[i * a.sy];
+    z = a.z[i * a.sz];
   }
-  w = a.w ? a.w[i * a.sw] : 1.0f;
+  w = hw ? a.w[i * a.sw] : 1.0f;
+  px = (x - a.xmin) * a.inv;
+  py = (y - a.ymin) * a.inv;
+  pz = (z - a.zmin) * a.inv;
 }
 
 __device__ __forceinline__ bool pos_ok(float px, float py, float pz) {
@@ -275,23 +286,32 @@ __device__ __forceinline__ void ref_corners_on_x_regular(const ScatterArgs& a, f
 }
 
 // Sorted particle records: float4(xpos, ypos, zpos, w) when weights are given, else
-// float3(xpos, ypos, zpos) packed at 12 bytes (w = 1).
+// float3(xpos, ypos, zpos) packed at 12 bytes (w = 1).  Record arrays hold 16 bytes per
+// particle, so a 16-byte read of the last packed record stays inside the array.
 __device__ __forceinline__ void store_record(void* sorted, size_t i, bool has_w, float px, float py, float pz,
                                              float w) {
-  if (has_w) {
-    reinterpret_cast<float4*>(sorted)[i] = make_float4(px, py, pz, w);
-  } else {
-    float* p = reinterpret_cast<float*>(sorted) + 3 * i;
-    p[0] = px;
-    p[1] = py;
-    p[2] = pz;
-  }
+  typedef float f3 __attribute__((ext_vector_type(3), aligned(4)));
+  float* p = static_cast<float*>(sorted) + i * (has_w ? 4 : 3);
+  const f3 v = {px, py, pz};
+  *reinterpret_cast<f3*>(p) = v;
+  if (has_w) p[3] = w;
 }
 
+// one 16-byte load at the record's stride for both layouts: no branch between the loads of
+// several records.  The fourth word of a packed record belongs to the next one: callers
+// that defer the weight select (record_weight) past their other loads keep every load in
+// flight.
+__device__ __forceinline__ float4 load_record_raw(const void* sorted, size_t i, bool has_w) {
+  typedef float f4 __attribute__((ext_vector_type(4), aligned(4)));
+  const float* p = static_cast<const float*>(sorted) + i * (has_w ? 4 : 3);
+  const f4 v = *reinterpret_cast<const f4*>(p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float record_weight(const float4& r, bool has_w) { return has_w ? r.w : 1.0f; }
 __device__ __forceinline__ float4 load_record(const void* sorted, size_t i, bool has_w) {
-  if (has_w) return reinterpret_cast<const float4*>(sorted)[i];
-  const float* p = reinterpret_cast<const float*>(sorted) + 3 * i;
-  return make_float4(p[0], p[1], p[2], 1.0f);
+  float4 r = load_record_raw(sorted, i, has_w);
+  r.w = record_weight(r, has_w);
+  return r;
 }
 
 // Row-bundle tiles: tile (xt, yt) owns cells x = xt, y in [yt*ty, yt*ty + ty), all z.

@@ -2,18 +2,28 @@
 // textbook CIC and TSC.
 //
 // Sorted owner-computes deposit on row-bundle tiles (1 x ty x n cells, full z rows):
-//   1. count:  each block takes a chunk of <= 65535 particles and histograms their keys
-//              (tile, y-class) in LDS (16-bit counters packed in pairs); the block's row of
-//              counts goes to a [blocks][keys] matrix.  No global atomics.
+//   1. count:  each block takes a chunk of particles (one block per CU) and histograms their
+//              keys (tile, y-class) in LDS, or in its row of the count matrix in HBM when
+//              the keys do not fit; the block's row of counts goes to a [blocks][keys]
+//              matrix.
 //   2. scan:   a column scan of the matrix gives every (block, key) its offset inside the
 //              key; an exclusive scan of the key totals gives the key starts.
-//   3. place:  the blocks re-read their chunk and write float4(xpos, ypos, zpos, w) records
-//              at key start + block offset + LDS rank.
+//   3. place, in two passes.  Keys are x-plane major, so the records of one x plane fill
+//              one contiguous range of the sorted order.
+//      a. coarse: the blocks re-read their chunk and write each record into its x plane's
+//              range (block offset = sum of the block's key offsets over the plane, plus an
+//              LDS rank).  A block appends to at most n + 1 runs, so the partly written
+//              128-byte lines of all blocks fit the L2 and leave it whole.
+//      b. fine:  blocks take consecutive segments of the plane-grouped records, histogram
+//              their keys over the segment's window of planes in LDS, reserve one run per
+//              key with a global atomic and write each key's records as one run.
+//              Owner-computes deposits sum in fixed point, so the order inside a key
+//              (atomic order) does not change any result.
 //   4. deposit: one block per tile stages its ty full rows in LDS (zeros or mesh_in), walks
 //              its own buckets plus the neighbour buckets that spill into it (the whole
 //              x - 1 / x + 1 tiles, the boundary-row classes of the y neighbours), adds every
-//              stencil corner that falls inside with LDS float atomics and writes each row
-//              once with coalesced stores: no global atomics and no zero fill.
+//              stencil corner that falls inside with LDS atomics (64-bit fixed point) and
+//              writes each row once with coalesced stores: no global atomics, no zero fill.
 //   5. atomic: the (normally empty) irregular bucket is added with global atomics.
 // A direct mode (tile[0] == 0) adds every particle with global atomics instead.
 #include <mutex>
@@ -37,7 +47,7 @@ __global__ void k_copy_or_zero(const float* src, float* dst, int64_t n) {  // sr
 }
 
 constexpr int CNT_NT = 1024;
-constexpr int CNT_UNROLL = 4;
+constexpr int CNT_UNROLL = 8;
 
 // Batched wrappers: blockIdx.y selects the job; blocks beyond the job's own extent return
 // at once (the whole block, so no barrier is left waiting).
@@ -48,12 +58,20 @@ constexpr int CNT_UNROLL = 4;
 // ------------------------------------------------------------------------------------
 // 1. count: LDS histogram of one chunk -> matrix row; FP64 weight sum per block
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
+struct CountShared {
+  double red[CNT_NT];
+  unsigned wmax_bits;
+};
+
+// AOS / W: the job's input layout and weights, fixed per instantiation (no branch between the
+// loads of a thread's particles)
+template <int AOS, int W>
+__device__ __forceinline__ void bucket_count_body(const Job& j, int bx, CountShared& sh) {
   // chunks of <= 65535 particles: key k -> 16-bit half (k & 1) of word k >> 1 (half the
   // LDS); larger chunks: one 32-bit counter per key
   extern __shared__ unsigned hist[];
-  __shared__ double red[CNT_NT];
-  __shared__ unsigned wmax_bits;
+  double* red = sh.red;
+  unsigned& wmax_bits = sh.wmax_bits;
   const ScatterArgs a = j.a;
   const TileGeom g = j.g;
   const int tid = threadIdx.x;
@@ -78,11 +96,12 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
   // the first key is computed
   for (int64_t i0 = lo + tid; i0 < hi; i0 += (int64_t)CNT_NT * CNT_UNROLL) {
     float px[CNT_UNROLL], py[CNT_UNROLL], pz[CNT_UNROLL], w[CNT_UNROLL];
+    // indexes clamped to the chunk: the loads need no branch (the extra ones repeat the
+    // last particle and are not counted)
 #pragma unroll
-    for (int u = 0; u < CNT_UNROLL; ++u) {
-      const int64_t i = i0 + (int64_t)u * CNT_NT;
-      if (i < hi) load_particle(a, i, px[u], py[u], pz[u], w[u]);
-    }
+    for (int u = 0; u < CNT_UNROLL; ++u)
+      load_particle<AOS, W>(a, min(i0 + (int64_t)u * CNT_NT, hi - 1), px[u], py[u], pz[u], w[u]);
+    __builtin_amdgcn_sched_barrier(0);   // every load issued before the first key
 #pragma unroll
     for (int u = 0; u < CNT_UNROLL; ++u) {
       if (i0 + (int64_t)u * CNT_NT >= hi) continue;
@@ -104,10 +123,23 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx) {
   const double t = block_sum_f64(ws, red, CNT_NT);
   if (tid == 0) j.wsum[bx] = t;
 }
-__global__ void __launch_bounds__(CNT_NT) k_bucket_count(Job j) { bucket_count_body(j, blockIdx.x); }
+template <int AOS, int W>
+__global__ void __launch_bounds__(CNT_NT) k_bucket_count(Job j) {
+  __shared__ CountShared sh;
+  bucket_count_body<AOS, W>(j, blockIdx.x, sh);
+}
 __global__ void __launch_bounds__(CNT_NT) k_bucket_count_jobs(const Job* __restrict__ jobs) {
+  __shared__ CountShared sh;
   PK_JOB_OR_RETURN(jobs, j.g.nblk);
-  bucket_count_body(j, blockIdx.x);
+  // block-uniform choice of the instantiation for this job's layout and weights
+  const bool w = j.a.w != nullptr;
+  if (j.a.aos) {
+    if (w) bucket_count_body<1, 1>(j, blockIdx.x, sh);
+    else bucket_count_body<1, 0>(j, blockIdx.x, sh);
+  } else {
+    if (w) bucket_count_body<0, 1>(j, blockIdx.x, sh);
+    else bucket_count_body<0, 0>(j, blockIdx.x, sh);
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -157,7 +189,10 @@ __device__ __forceinline__ void col_scan_body(const Job& j, int bx) {
     }
     carry += total;
   }
-  if (on && sub == 0) j.totals[k] = (int)carry;
+  if (on && sub == 0) {
+    j.totals[k] = (int)carry;
+    j.koff[k] = 0u;   // the fine placing pass counts each key's records from zero
+  }
 }
 __global__ void __launch_bounds__(COL_NT) k_col_scan(Job j) { col_scan_body(j, blockIdx.x); }
 __global__ void __launch_bounds__(COL_NT) k_col_scan_jobs(const Job* __restrict__ jobs) {
@@ -295,49 +330,280 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_add_jobs(const Job* __restrict
 }
 
 // ------------------------------------------------------------------------------------
-// 3. place: key start + block offset + LDS rank
+// 3. place (both passes): rounds of records are ranked per destination run in LDS, staged
+// there in destination order and written by consecutive lanes to consecutive addresses, so
+// a wave's stores touch a few runs instead of one line per lane
 // ------------------------------------------------------------------------------------
-constexpr int PLACE_UNROLL = 4;
 
-__device__ __forceinline__ void bucket_place_body(const Job& j, int bx) {
-  extern __shared__ unsigned dest[];   // next write position of every key for this block
+// Exclusive scan of the counts a[0, len) in LDS by the whole block of NT threads (each takes
+// a contiguous span): f(i, count, prefix) for every entry, in the thread that owns it.
+// Returns the grand total.  Every thread must call it (it holds a barrier).
+template <int NT, class F>
+__device__ __forceinline__ unsigned block_scan_apply(const unsigned* a, int len, unsigned* wtot, F&& f) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int per = (len + NT - 1) / NT;
+  const int i0 = min(tid * per, len), i1 = min(i0 + per, len);
+  unsigned own = 0;
+  for (int i = i0; i < i1; ++i) own += a[i];
+  unsigned incl = own;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  unsigned pre = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < NT / 64; ++k) {
+    const unsigned t = wtot[k];
+    if (k < wv) pre += t;
+    total += t;
+  }
+  unsigned e = pre + incl - own;
+  for (int i = i0; i < i1; ++i) {
+    const unsigned c = a[i];
+    f(i, c, e);
+    e += c;
+  }
+  return total;
+}
+
+// x plane of a key: keys run (x, y tile, class) with x slowest, and the irregular key
+// (nkeys - 1 = n kpp) makes group n
+__device__ __forceinline__ unsigned key_plane(int key, unsigned kpp) { return (unsigned)key / kpp; }
+
+// ------------------------------------------------------------------------------------
+// 3a. coarse place: x-plane start + block offset inside the plane + LDS rank
+// ------------------------------------------------------------------------------------
+constexpr int PLACE_PER = 7;                     // most particles per thread and round
+constexpr size_t PLACE_LDS = 150 * 1024;         // tables + staging of the coarse pass
+// particles per round: 20 staged bytes each (x, y, z, w, destination) next to three tables
+// of n + 1 words, a multiple of the block size
+__host__ __device__ constexpr int place_round(int n) {
+  return (int)((PLACE_LDS - 12 * (size_t)(n + 1)) / 20) / CNT_NT * CNT_NT > CNT_NT * PLACE_PER
+             ? CNT_NT * PLACE_PER
+             : (int)((PLACE_LDS - 12 * (size_t)(n + 1)) / 20) / CNT_NT * CNT_NT;
+}
+size_t place_lds(int n) { return 12 * ((size_t)n + 1) + 20 * (size_t)place_round(n); }
+
+template <int AOS, int W>
+__device__ __forceinline__ void bucket_place_body(const Job& j, int bx, unsigned* wtot) {
+  extern __shared__ unsigned lds[];
   const ScatterArgs a = j.a;
   const TileGeom g = j.g;
-  void* sorted = j.sorted;
   const int tid = threadIdx.x;
-  unsigned* row = j.mat + (size_t)bx * g.nkeys;   // this block's offset inside every key
-  if (!g.gtab) {
-    for (int k = tid; k < g.nkeys; k += CNT_NT) dest[k] = (unsigned)j.key_start[k] + row[k];
-    __syncthreads();
+  const int G = a.n + 1;            // x planes, then the irregular key
+  const int S = place_round(a.n);   // particles per round
+  unsigned* dest = lds;             // next write position of each plane for this block
+  unsigned* lcnt = lds + G;         // the round's count, then local start, of each plane
+  unsigned* delta = lds + 2 * G;    // global position - local slot, per plane
+  float* sx = reinterpret_cast<float*>(lds + 3 * G);
+  float* sy = sx + S;
+  float* sz = sy + S;
+  float* sw = sz + S;
+  unsigned* sd = reinterpret_cast<unsigned*>(sw + S);   // global position of each slot
+  const unsigned kpp = (unsigned)(g.nyt * g.ncls);
+  {
+    // plane start + the records of this plane in earlier blocks (the sum of this block's
+    // offsets over the plane's keys): one wave per plane
+    const unsigned* row = j.mat + (size_t)bx * g.nkeys;
+    const int wv = tid >> 6, ln = tid & 63;
+    for (int i = wv; i < G; i += CNT_NT / 64) {
+      const int k0 = i * (int)kpp, k1 = i < a.n ? k0 + (int)kpp : g.nkeys;
+      unsigned sum = 0;
+      for (int k = k0 + ln; k < k1; k += 64) sum += row[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+      if (ln == 0) dest[i] = (unsigned)j.key_start[k0] + sum;
+    }
   }
   const int64_t lo = (int64_t)bx * g.chunk;
   const int64_t hi = min(lo + (int64_t)g.chunk, a.np);
   const bool has_w = a.w != nullptr;
-  // PLACE_UNROLL independent particles per thread and iteration keep several loads in flight
-  for (int64_t i0 = lo + tid; i0 < hi; i0 += (int64_t)CNT_NT * PLACE_UNROLL) {
-    float px[PLACE_UNROLL], py[PLACE_UNROLL], pz[PLACE_UNROLL], w[PLACE_UNROLL];
+  // software pipeline: the next round's particles load while this round is ranked, staged
+  // and stored
+  float px[PLACE_PER], py[PLACE_PER], pz[PLACE_PER], w[PLACE_PER];
+  // indexes clamped to the chunk: no branch between the loads (the extra loads repeat the
+  // chunk's last particle and are not placed)
+  auto load_round = [&](int64_t r0, float* x, float* y, float* z, float* ww) {
 #pragma unroll
-    for (int u = 0; u < PLACE_UNROLL; ++u) {
-      const int64_t i = i0 + (int64_t)u * CNT_NT;
-      if (i < hi) load_particle(a, i, px[u], py[u], pz[u], w[u]);
-    }
+    for (int u = 0; u < PLACE_PER; ++u)
+      load_particle<AOS, W>(a, min(r0 + (int64_t)u * CNT_NT + tid, hi - 1), x[u], y[u], z[u], ww[u]);
+  };
+  if (lo >= hi) return;   // whole block: an empty chunk
+  load_round(lo, px, py, pz, w);
+  for (int64_t r0 = lo; r0 < hi; r0 += S) {   // block-uniform
+    int pl[PLACE_PER];
 #pragma unroll
-    for (int u = 0; u < PLACE_UNROLL; ++u) {
-      const int64_t i = i0 + (int64_t)u * CNT_NT;
-      if (i >= hi) continue;
-      const int key = row_tile_key(a, g, px[u], py[u], pz[u]);
-      if (key < 0) continue;
-      // HBM tables: the block's row (offsets after the column scan) advances with global
-      // atomics, each returning this particle's rank
-      const unsigned at = g.gtab ? (unsigned)j.key_start[key] + atomicAdd(&row[key], 1u) : atomicAdd(&dest[key], 1u);
-      store_record(sorted, at, has_w, px[u], py[u], pz[u], w[u]);
+    for (int u = 0; u < PLACE_PER; ++u) {
+      const int64_t i = r0 + (int64_t)u * CNT_NT + tid;
+      pl[u] = -1;
+      if (u * CNT_NT < S && i < hi) {
+        const int key = row_tile_key(a, g, px[u], py[u], pz[u]);
+        if (key >= 0) pl[u] = (int)key_plane(key, kpp);
+      }
     }
+    float nx[PLACE_PER], ny[PLACE_PER], nz[PLACE_PER], nw[PLACE_PER];
+    load_round(r0 + S, nx, ny, nz, nw);
+    for (int q = tid; q < G; q += CNT_NT) lcnt[q] = 0u;
+    __syncthreads();
+    unsigned rk[PLACE_PER];
+#pragma unroll
+    for (int u = 0; u < PLACE_PER; ++u)
+      if (pl[u] >= 0) rk[u] = atomicAdd(&lcnt[pl[u]], 1u);
+    __syncthreads();
+    const unsigned m = block_scan_apply<CNT_NT>(lcnt, G, wtot, [&](int p, unsigned c, unsigned e) {
+      lcnt[p] = e;
+      if (c) {   // this thread alone owns plane p here
+        const unsigned b = dest[p];
+        dest[p] = b + c;
+        delta[p] = b - e;
+      }
+    });
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PLACE_PER; ++u)
+      if (pl[u] >= 0) {
+        const unsigned slot = lcnt[pl[u]] + rk[u];
+        sx[slot] = px[u];
+        sy[slot] = py[u];
+        sz[slot] = pz[u];
+        sw[slot] = w[u];
+        sd[slot] = delta[pl[u]] + slot;
+      }
+    __syncthreads();
+    for (int q = tid; q < (int)m; q += CNT_NT) store_record(j.staged, sd[q], has_w, sx[q], sy[q], sz[q], sw[q]);
+#pragma unroll
+    for (int u = 0; u < PLACE_PER; ++u) {
+      px[u] = nx[u];
+      py[u] = ny[u];
+      pz[u] = nz[u];
+      w[u] = nw[u];
+    }
+    __syncthreads();   // the staging is read before the next round overwrites it
   }
 }
-__global__ void __launch_bounds__(CNT_NT) k_bucket_place(Job j) { bucket_place_body(j, blockIdx.x); }
+template <int AOS, int W>
+__global__ void __launch_bounds__(CNT_NT) k_bucket_place(Job j) {
+  __shared__ unsigned wtot[CNT_NT / 64];
+  bucket_place_body<AOS, W>(j, blockIdx.x, wtot);
+}
 __global__ void __launch_bounds__(CNT_NT) k_bucket_place_jobs(const Job* __restrict__ jobs) {
+  __shared__ unsigned wtot[CNT_NT / 64];
   PK_JOB_OR_RETURN(jobs, j.g.nblk);
-  bucket_place_body(j, blockIdx.x);
+  const bool w = j.a.w != nullptr;
+  if (j.a.aos) {
+    if (w) bucket_place_body<1, 1>(j, blockIdx.x, wtot);
+    else bucket_place_body<1, 0>(j, blockIdx.x, wtot);
+  } else {
+    if (w) bucket_place_body<0, 1>(j, blockIdx.x, wtot);
+    else bucket_place_body<0, 0>(j, blockIdx.x, wtot);
+  }
+}
+
+// the plain launches' instantiation for a job's layout and weights
+template <template <int, int> class Pick>
+auto pick_aw(const ScatterArgs& a) -> decltype(Pick<0, 0>::k) {
+  const bool w = a.w != nullptr;
+  if (a.aos) return w ? Pick<1, 1>::k : Pick<1, 0>::k;
+  return w ? Pick<0, 1>::k : Pick<0, 0>::k;
+}
+template <int AOS, int W> struct CountK { static constexpr void (*k)(Job) = k_bucket_count<AOS, W>; };
+template <int AOS, int W> struct PlaceK { static constexpr void (*k)(Job) = k_bucket_place<AOS, W>; };
+
+// ------------------------------------------------------------------------------------
+// 3b. fine place: a segment of the plane-grouped records -> one run per (segment, key)
+// ------------------------------------------------------------------------------------
+constexpr int FINE_NT = 256;                      // small blocks, several per CU: their
+constexpr int FINE_PER = 6;                       // load, rank and store phases overlap
+constexpr int FINE_SEG = FINE_NT * FINE_PER;      // records per block
+constexpr int FINE_WIN = 1024;                    // keys of the LDS window
+constexpr size_t FINE_LDS = (size_t)FINE_SEG * 20;   // staging: x, y, z, w, destination
+
+int fine_place_blocks(int64_t np) { return (int)((np > 0 ? np + FINE_SEG - 1 : FINE_SEG) / FINE_SEG); }
+
+__device__ __forceinline__ void fine_place_body(const Job& j, int bx) {
+  extern __shared__ unsigned lds[];
+  __shared__ unsigned cnt[FINE_WIN];     // per key of the window: count, then local start
+  __shared__ unsigned delta[FINE_WIN];   // global position - local slot
+  __shared__ unsigned wtot[FINE_NT / 64];
+  __shared__ int win[2];
+  const ScatterArgs a = j.a;
+  const TileGeom g = j.g;
+  const int tid = threadIdx.x;
+  const int64_t total = j.key_start[g.nkeys];   // records kept
+  const int64_t s0 = (int64_t)bx * FINE_SEG;
+  if (s0 >= total) return;   // whole block
+  const int64_t s1 = min(s0 + (int64_t)FINE_SEG, total);
+  const bool has_w = a.w != nullptr;
+  const unsigned kpp = (unsigned)(g.nyt * g.ncls);
+  float* sx = reinterpret_cast<float*>(lds);
+  float* sy = sx + FINE_SEG;
+  float* sz = sy + FINE_SEG;
+  float* sw = sz + FINE_SEG;
+  unsigned* sd = reinterpret_cast<unsigned*>(sw + FINE_SEG);
+  // the records are grouped by plane, so the first and last records bound the segment's planes
+  if (tid < 2) {
+    const float4 r = load_record(j.staged, (size_t)(tid == 0 ? s0 : s1 - 1), has_w);
+    win[tid] = (int)key_plane(row_tile_key(a, g, r.x, r.y, r.z), kpp);
+  }
+  float4 r[FINE_PER];
+  int key[FINE_PER];
+#pragma unroll
+  for (int u = 0; u < FINE_PER; ++u)   // clamped: no branch between the loads
+    r[u] = load_record_raw(j.staged, (size_t)min(s0 + tid + (int64_t)u * FINE_NT, s1 - 1), has_w);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int u = 0; u < FINE_PER; ++u) {
+    const int64_t i = s0 + tid + (int64_t)u * FINE_NT;
+    r[u].w = record_weight(r[u], has_w);
+    key[u] = i < s1 ? row_tile_key(a, g, r[u].x, r[u].y, r[u].z) : -1;   // the key the coarse pass saw
+  }
+  __syncthreads();
+  const int kbase = win[0] * (int)kpp;
+  const int wlen = min((win[1] + 1) * (int)kpp, g.nkeys) - kbase;
+  if (wlen > FINE_WIN) {
+    // a sparse segment spanning too many planes (small catalogues): one global atomic each
+#pragma unroll
+    for (int u = 0; u < FINE_PER; ++u)
+      if (key[u] >= 0) {
+        const unsigned at = (unsigned)j.key_start[key[u]] + atomicAdd(&j.koff[key[u]], 1u);
+        store_record(j.sorted, at, has_w, r[u].x, r[u].y, r[u].z, r[u].w);
+      }
+    return;
+  }
+  for (int q = tid; q < wlen; q += FINE_NT) cnt[q] = 0u;
+  __syncthreads();
+  unsigned rank[FINE_PER];
+#pragma unroll
+  for (int u = 0; u < FINE_PER; ++u)
+    if (key[u] >= 0) rank[u] = atomicAdd(&cnt[key[u] - kbase], 1u);
+  __syncthreads();
+  const unsigned m = block_scan_apply<FINE_NT>(cnt, wlen, wtot, [&](int q, unsigned c, unsigned e) {
+    cnt[q] = e;
+    if (c) delta[q] = (unsigned)j.key_start[kbase + q] + atomicAdd(&j.koff[kbase + q], c) - e;
+  });
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < FINE_PER; ++u)
+    if (key[u] >= 0) {
+      const int q = key[u] - kbase;
+      const unsigned slot = cnt[q] + rank[u];
+      sx[slot] = r[u].x;
+      sy[slot] = r[u].y;
+      sz[slot] = r[u].z;
+      sw[slot] = r[u].w;
+      sd[slot] = delta[q] + slot;
+    }
+  __syncthreads();
+  for (int q = tid; q < (int)m; q += FINE_NT) store_record(j.sorted, sd[q], has_w, sx[q], sy[q], sz[q], sw[q]);
+}
+__global__ void __launch_bounds__(FINE_NT) k_fine_place(Job j) { fine_place_body(j, blockIdx.x); }
+__global__ void __launch_bounds__(FINE_NT) k_fine_place_jobs(const Job* __restrict__ jobs) {
+  const Job& j = jobs[blockIdx.y];
+  if (!j.active) return;
+  fine_place_body(j, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -549,7 +815,6 @@ size_t hist_lds(const TileGeom& g) {
   if (g.gtab) return 0;
   return g.chunk <= 65535 ? (size_t)((g.nkeys + 1) / 2) * 4 : (size_t)g.nkeys * 4;
 }
-size_t dest_lds(const TileGeom& g) { return g.gtab ? 0 : (size_t)g.nkeys * 4; }
 }  // namespace
 
 size_t carve_scatter(Job& j, bool direct, void* base) {
@@ -571,7 +836,9 @@ size_t carve_scatter(Job& j, bool direct, void* base) {
     j.wsum = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
     j.wmax = static_cast<double*>(take((size_t)g.nblk * sizeof(double)));
     j.aux = static_cast<double*>(take(AUX_LEN * sizeof(double)));
+    j.koff = static_cast<unsigned*>(take((size_t)g.nkeys * sizeof(unsigned)));
     j.sorted = take((size_t)(j.a.np > 0 ? j.a.np : 1) * sizeof(float4));
+    j.staged = take((size_t)(j.a.np > 0 ? j.a.np : 1) * sizeof(float4));
     j.tile_mass = static_cast<double*>(take((size_t)g.ntiles * sizeof(double)));
   }
   return off;
@@ -620,9 +887,10 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
   {
     StageTimer t(ST_SORT_COUNT, s);
     const size_t lds = hist_lds(g);
-    int e = allow_lds(k_bucket_count, lds);
+    const auto kc = pick_aw<CountK>(a);
+    int e = allow_lds(kc, lds);
     if (e) return e;
-    hipLaunchKernelGGL(k_bucket_count, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
+    hipLaunchKernelGGL(kc, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
     e = check_hip(hipGetLastError(), "bucket count launch");
     if (e) return e;
   }
@@ -642,11 +910,20 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
   }
   {
     StageTimer t(ST_SORT_PLACE, s);
-    const size_t lds = dest_lds(g);
-    int e = allow_lds(k_bucket_place, lds);
+    const size_t lds = place_lds(a.n);
+    const auto kp = pick_aw<PlaceK>(a);
+    int e = allow_lds(kp, lds);
     if (e) return e;
-    hipLaunchKernelGGL(k_bucket_place, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
+    hipLaunchKernelGGL(kp, dim3(g.nblk), dim3(CNT_NT), lds, s, j);
     e = check_hip(hipGetLastError(), "bucket place launch");
+    if (e) return e;
+  }
+  {
+    StageTimer t(ST_SORT_FINE, s);
+    int e = allow_lds(k_fine_place, FINE_LDS);
+    if (e) return e;
+    hipLaunchKernelGGL(k_fine_place, dim3(fine_place_blocks(a.np)), dim3(FINE_NT), FINE_LDS, s, j);
+    e = check_hip(hipGetLastError(), "fine place launch");
     if (e) return e;
   }
   if (!mesh_out) return PK_OK;   // the caller deposits (fused with the z pass)
@@ -699,12 +976,20 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
     int e = check_hip(hipGetLastError(), "batch scan launch");
     if (e) return e;
   }
-  StageTimer t(ST_SORT_PLACE, s);
-  const size_t lds = (size_t)d.lds_keys * 4;
-  int e = allow_lds(k_bucket_place_jobs, lds);
+  {
+    StageTimer t(ST_SORT_PLACE, s);
+    const size_t lds = place_lds(d.n);
+    int e = allow_lds(k_bucket_place_jobs, lds);
+    if (e) return e;
+    hipLaunchKernelGGL(k_bucket_place_jobs, dim3(d.max_nblk, nj), dim3(CNT_NT), lds, s, jobs);
+    e = check_hip(hipGetLastError(), "batch bucket place launch");
+    if (e) return e;
+  }
+  StageTimer t(ST_SORT_FINE, s);
+  int e = allow_lds(k_fine_place_jobs, FINE_LDS);
   if (e) return e;
-  hipLaunchKernelGGL(k_bucket_place_jobs, dim3(d.max_nblk, nj), dim3(CNT_NT), lds, s, jobs);
-  return check_hip(hipGetLastError(), "batch bucket place launch");
+  hipLaunchKernelGGL(k_fine_place_jobs, dim3(d.max_fblk, nj), dim3(FINE_NT), FINE_LDS, s, jobs);
+  return check_hip(hipGetLastError(), "batch fine place launch");
 }
 
 int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s) {
