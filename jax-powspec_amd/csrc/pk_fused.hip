@@ -89,6 +89,9 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   if (tid < 2) tsum[tid] = 0ull;
   const float scale = (float)j.aux[0];
   const double inv_scale = j.aux[1];
+  int inv_exp;   // inv_scale = 2^inv_exp
+  (void)frexp(inv_scale, &inv_exp);
+  inv_exp -= 1;
   const float c0 = (float)j.stats[3];
   float2* spec = j.spec;
   const int* __restrict__ key_start = j.key_start;
@@ -214,13 +217,29 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
         const int task = tid + u * NT;
         if (TASKS % NT == 0 || task < TASKS) {
           const int jj = task / C, c = task % C;
+          long long q[2 * R0];
+          bool fit = true;
 #pragma unroll
           for (int r = 0; r < R0; ++r) {
             const int p = jj + r * B0;   // complex point p = cells z = 2p, 2p + 1
-            const long long q0 = (long long)acc[(2 * p) * C + c];
-            const long long q1 = (long long)acc[(2 * p + 1) * C + c];
-            isum += q0 + q1;
-            v[u][r] = make_float2((float)((double)q0 * inv_scale) - c0, (float)((double)q1 * inv_scale) - c0);
+            q[2 * r] = (long long)acc[(2 * p) * C + c];
+            q[2 * r + 1] = (long long)acc[(2 * p + 1) * C + c];
+            isum += q[2 * r] + q[2 * r + 1];
+            fit = fit && (long long)(int)q[2 * r] == q[2 * r] && (long long)(int)q[2 * r + 1] == q[2 * r + 1];
+          }
+          // rho - c: float(q) 2^inv_exp equals float(q inv_scale) (scaling by a power of two
+          // commutes with the rounding), so when every cell of the wave is below 2^31 in
+          // magnitude (nearly always) they convert through int32, not int64 -> double -> float
+          if (__builtin_amdgcn_ballot_w64(!fit) == 0) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r)
+              v[u][r] = make_float2(ldexpf((float)(int)q[2 * r], inv_exp) - c0,
+                                    ldexpf((float)(int)q[2 * r + 1], inv_exp) - c0);
+          } else {
+#pragma unroll
+            for (int r = 0; r < R0; ++r)
+              v[u][r] = make_float2((float)((double)q[2 * r] * inv_scale) - c0,
+                                    (float)((double)q[2 * r + 1] * inv_scale) - c0);
           }
         }
       }
