@@ -34,6 +34,7 @@ struct TileGeom {
   int chunk;    // particles per counting block (<= 65535: packed 16-bit LDS counters)
   int nblk;     // counting blocks
   int fixed;    // 1: deposit into 64-bit fixed-point LDS cells (ty * n * 8 bytes fit), 0: float
+  unsigned ty_mul;   // ceil(2^32 / ty): y / ty = umulhi(y, ty_mul) for 0 <= y < n
   int gtab;     // 1: the count and place kernels keep their per-key tables in HBM (the
                 // block's row of the count matrix, global atomics): nkeys too large for LDS
 };

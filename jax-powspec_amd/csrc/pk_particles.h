@@ -319,21 +319,27 @@ __device__ __forceinline__ float4 load_record(const void* sorted, size_t i, bool
 // into the neighbouring tile (CIC variants: 1 = last row; TSC: 1 = first, 2 = last row).
 // The x neighbours are read whole (one x-layer per tile).  Last key: irregular particles
 // (reference stencil with an out-of-range base index); -1: dropped.
-__device__ __forceinline__ int row_tile_key(const ScatterArgs& a, const TileGeom& g, float px, float py, float pz) {
+// plane (optional): the key's x plane hx, or n for the irregular key (keys are x-plane major)
+__device__ __forceinline__ int row_tile_key(const ScatterArgs& a, const TileGeom& g, float px, float py, float pz,
+                                            int* plane = nullptr) {
   if (!pos_ok(px, py, pz)) return -1;
   const int n = a.n;
   if (a.stencil == PK_STENCIL_CIC_REF) {
     const int ix = (int)px, iy = (int)py, iz = (int)pz;
     const int bx = ix < 0 ? ix + n : ix, by = iy < 0 ? iy + n : iy, bz = iz < 0 ? iz + n : iz;
-    if (bx < 0 || bx >= n || by < 0 || by >= n || bz < 0 || bz >= n) return g.nkeys - 1;
+    if (bx < 0 || bx >= n || by < 0 || by >= n || bz < 0 || bz >= n) {
+      if (plane) *plane = n;
+      return g.nkeys - 1;
+    }
   }
   const int hx = home_cell(px, n, a.stencil);
   const int hy = home_cell(py, n, a.stencil);
-  const int yt = hy / g.ty;
+  const int yt = (int)__umulhi((unsigned)hy, g.ty_mul);   // hy / ty
   const int ly = hy - yt * g.ty;
   int cls;
   if (a.stencil == PK_STENCIL_TSC) cls = (ly == 0) ? 1 : ((ly == g.ty - 1) ? 2 : 0);
   else cls = (ly == g.ty - 1) ? 1 : 0;
+  if (plane) *plane = hx;
   return (hx * g.nyt + yt) * g.ncls + cls;
 }
 

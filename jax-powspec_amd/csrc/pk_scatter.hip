@@ -369,10 +369,6 @@ __device__ __forceinline__ unsigned block_scan_apply(const unsigned* a, int len,
   return total;
 }
 
-// x plane of a key: keys run (x, y tile, class) with x slowest, and the irregular key
-// (nkeys - 1 = n kpp) makes group n
-__device__ __forceinline__ unsigned key_plane(int key, unsigned kpp) { return (unsigned)key / kpp; }
-
 // ------------------------------------------------------------------------------------
 // 3a. coarse place: x-plane start + block offset inside the plane + LDS rank
 // ------------------------------------------------------------------------------------
@@ -440,8 +436,8 @@ __device__ __forceinline__ void bucket_place_body(const Job& j, int bx, unsigned
       const int64_t i = r0 + (int64_t)u * CNT_NT + tid;
       pl[u] = -1;
       if (u * CNT_NT < S && i < hi) {
-        const int key = row_tile_key(a, g, px[u], py[u], pz[u]);
-        if (key >= 0) pl[u] = (int)key_plane(key, kpp);
+        int plane;
+        if (row_tile_key(a, g, px[u], py[u], pz[u], &plane) >= 0) pl[u] = plane;
       }
     }
     float nx[PLACE_PER], ny[PLACE_PER], nz[PLACE_PER], nw[PLACE_PER];
@@ -546,7 +542,9 @@ __device__ __forceinline__ void fine_place_body(const Job& j, int bx) {
   // the records are grouped by plane, so the first and last records bound the segment's planes
   if (tid < 2) {
     const float4 r = load_record(j.staged, (size_t)(tid == 0 ? s0 : s1 - 1), has_w);
-    win[tid] = (int)key_plane(row_tile_key(a, g, r.x, r.y, r.z), kpp);
+    int plane = 0;
+    (void)row_tile_key(a, g, r.x, r.y, r.z, &plane);
+    win[tid] = plane;
   }
   float4 r[FINE_PER];
   int key[FINE_PER];
@@ -765,6 +763,8 @@ int make_tile_geom(int n, int64_t np, int stencil, const int* tile, TileGeom* g,
   }
   g->ty = ty;
   g->nyt = n / ty;
+  // exact for y * ty < 2^32 (y < n <= 4096): the multiplier exceeds 2^32 / ty by under 1
+  g->ty_mul = (unsigned)((0xffffffffull + (unsigned long long)ty) / (unsigned long long)ty);
   g->ncls = stencil == PK_STENCIL_TSC ? 3 : 2;
   const long ntiles = (long)n * g->nyt;
   const long nkeys = ntiles * g->ncls + 1;
