@@ -106,11 +106,14 @@ STAGE_KERNELS = {
 }
 
 
-def stage_kernel(stage, n, fused):
+def stage_kernel(stage, n, fused, stencil=0):
     if stage == "deposit":
-        # default row-bundle height of the fused kernel (largest of 32, 24, 16 dividing n)
+        # default row-bundle height of the fused kernel (largest of 32, 24, 16 dividing n,
+        # 16 at 768 and 1024) and the stencil it is instantiated for (0: reference CIC)
         ty = n if n <= 32 else (32 if n % 32 == 0 else (24 if n % 24 == 0 else 16))
-        return f"k_fused_deposit_zpass<{n}, {ty}>" if fused else "k_rows_deposit"
+        if ty * n * 8 > 131072:
+            ty = 16
+        return f"k_fused_deposit_zpass<{n}, {ty}, {stencil}>" if fused else "k_rows_deposit"
     return STAGE_KERNELS.get(stage, stage)
 
 
