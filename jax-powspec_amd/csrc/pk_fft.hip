@@ -353,6 +353,30 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       if (N % RSTEP == 0 || r < N) buf[r * PS + lc] = v[k];
     }
   };
+  // First radix stage in registers: when RSTEP divides N / R0, a lane's rows lr0 + k RSTEP
+  // are the inputs (rows j + r N / R0) of whole first-stage butterflies j = lr0 + m RSTEP, so
+  // it transforms them and stores the stage's outputs (rows j R0 + r): the tile store, the
+  // first stage's LDS read and one barrier fold into a single pass.
+  constexpr int R0 = radix_at(N, 0);
+  constexpr int B0 = N / R0;
+  constexpr bool S0REG = N % RSTEP == 0 && B0 % RSTEP == 0;
+  auto store_stage0 = [&](const float2* v) {
+    constexpr int MB = S0REG ? B0 / RSTEP : 1;   // butterflies per lane
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      float2 x[R0];
+#pragma unroll
+      for (int r = 0; r < R0; ++r) x[r] = v[mb + r * MB];   // row lr0 + (mb + r MB) RSTEP
+      DFT<R0>::run(x);
+      const int jb = lr0 + mb * RSTEP;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) buf[(jb * R0 + r) * PS + lc] = x[r];
+    }
+  };
+  auto store_first = [&](const float2* v) {
+    if constexpr (S0REG) store_stage0(v);
+    else store_tile(v);
+  };
   // Strips narrower than a 128-byte line (QUAD with HW < 16): the blocks of work items t and
   // t + 8 run on the same XCD at about the same time, so they take the two strips of each
   // line (tiles 2P and 2P + 1) and the second read finds the line in L2.
@@ -384,17 +408,18 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     int thr_next = b < nb ? thr_at(b + 1) : 0x7fffffff;
     const float wyz = wt(kyt) * wt(kz);
     if constexpr (XPF) {
-      store_tile(xpre);
+      store_first(xpre);
     } else {
       float2 v[XPER];
       load_tile(tile, v);
-      store_tile(v);
+      store_first(v);
     }
     __syncthreads();
     if constexpr (XPF) {
       if (item + nbx < NTILES) load_tile(tile_of(item + nbx), xpre);
     }
-    lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
+    if constexpr (S0REG) FFTStages<N, 1, R0, C, PS, 1, NT, true, 1, 0>::run(buf, tw);
+    else lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
     // tile column A of the task, and B for a quad
     const bool quad = QUAD && s >= 1 && s < HW;
     const int cA = s < LPR ? s : 0;
@@ -416,16 +441,20 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
         atomicAdd(g + 3, (double)cnt);   // exact: integers below 2^53
       }
     };
+    // Each row reads its entries unconditionally (rows clamped to H, columns B = A for
+    // singles) and masks what it adds, so the loop has no divergent branch but the flush of
+    // a finished bin; masked entries add exact zeros, so the sums keep their order.
+    const float fquad = quad ? 1.0f : 0.0f;
 #pragma unroll 1
     for (int i = 0; i < SL; ++i) {
       const int q = qa + i;
-      if (q >= qe) break;
+      const bool valid = q < qe;
       const int m = q * q + r2;
-      while (m >= thr_next) {   // the bin moves forward (rarely more than one step)
+      while (valid && m >= thr_next) {   // the bin moves forward (rarely more than one step)
         ++b;
         thr_next = b < nb ? thr_at(b + 1) : 0x7fffffff;
       }
-      const int slot = b >= 0 && b < nb ? b : -1;
+      const int slot = !valid ? cur : (b >= 0 && b < nb ? b : -1);
       if (slot != cur) {
         flush();
         cur = slot;
@@ -434,31 +463,33 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
       }
       // the k = 0 mode's power is added by the finalize from the FP64 sum of the mesh
       // (mu = 0 there, :37), and its count too
-      if (slot < 0 || m == 0) continue;
-      const bool pair = q != 0 && q != H;
-      const int qm = q == 0 ? 0 : N - q;
-      const float2 av = buf[q * PS + cA];
+      const bool use = valid && slot >= 0 && m != 0;
+      const int qc = min(q, H);
+      const bool pair = qc != 0 && qc != H;
+      const int qm = qc == 0 ? 0 : N - qc;
+      const float2 av = buf[qc * PS + cA];
       const float2 bv = buf[qm * PS + cA];
-      float p2 = (av.x * av.x + av.y * av.y) + (pair ? (bv.x * bv.x + bv.y * bv.y) : 0.0f);
-      if (quad) {
-        const float2 cv = buf[q * PS + cB];
-        const float2 dv = buf[qm * PS + cB];
-        p2 += (cv.x * cv.x + cv.y * cv.y) + (pair ? (dv.x * dv.x + dv.y * dv.y) : 0.0f);
-      }
-      p2 *= wt(q) * wyz;
+      const float2 cv = buf[qc * PS + cB];
+      const float2 dv = buf[qm * PS + cB];
+      const float fpair = pair ? 1.0f : 0.0f;
+      float p2 = (av.x * av.x + av.y * av.y) + fpair * (bv.x * bv.x + bv.y * bv.y);
+      p2 += fquad * ((cv.x * cv.x + cv.y * cv.y) + fpair * (dv.x * dv.x + dv.y * dv.y));
+      p2 = use ? p2 * (wt(qc) * wyz) : 0.0f;
       // Legendre weights from integer numerators (mu^2 = kz^2 / m):
       //   L2 = (3 kz^2 - m) / (2 m),  L4 = (35 kz^4 - 30 kz^2 m + 3 m^2) / (8 m^2)
       // so L2 is exactly 0 where 3 kz^2 = m; 1 / m to within 1 ulp (hardware reciprocal);
-      // the L4 numerator in float has relative error ~1e-7 of its terms
-      const float rm = __builtin_amdgcn_rcpf((float)m);
-      const float mf = (float)m;
-      const float l2 = 0.5f * (float)(3 * kz2 - m) * rm;
-      const float num4 = fmaf((float)kz2, (float)(35 * kz2 - 30 * m), 3.0f * mf * mf);
+      // the L4 numerator in float has relative error ~1e-7 of its terms (m = 0, masked,
+      // takes m = 1 so that no infinity meets the zero weight)
+      const int mm = m > 0 ? m : 1;
+      const float rm = __builtin_amdgcn_rcpf((float)mm);
+      const float mf = (float)mm;
+      const float l2 = 0.5f * (float)(3 * kz2 - mm) * rm;
+      const float num4 = fmaf((float)kz2, (float)(35 * kz2 - 30 * mm), 3.0f * mf * mf);
       const float l4 = 0.125f * num4 * rm * rm;
       a0 += p2;
       a2 = fmaf(p2, l2, a2);
       a4 = fmaf(p2, l4, a4);
-      cnt += (quad ? 2 : 1) * (pair ? 2 : 1);
+      cnt += use ? (quad ? 2 : 1) * (pair ? 2 : 1) : 0;
     }
     flush();
     __syncthreads();
