@@ -351,10 +351,13 @@ int pk_populate_place(const float* field, const float* psi_z, double growth_rate
 /* --- per-stage timing (bench / profiling; no reference counterpart) ----------------
  * When enabled, each kernel stage is bracketed by HIP events on its own stream.
  * pk_timing_read synchronises those events, returns total ms and launch counts per stage
- * (arrays of PK_NUM_STAGES) and clears the record.
+ * (arrays of PK_NUM_STAGES) and clears the record.  pk_timing_select(stage) restricts the
+ * events to one stage (-1: every stage), so a timed loop carries two event records per
+ * estimate instead of two per stage.
  */
 #define PK_NUM_STAGES 15
 int pk_timing_enable(int on);
+int pk_timing_select(int stage);
 int pk_timing_read(double* ms, int64_t* launches, int nstages);
 const char* pk_stage_name(int stage);
 

@@ -95,6 +95,7 @@ SIGNATURES = [
     ("pk_populate_count", _i, [_vp, _i, _d, _d, _d, C.c_uint64, _vp, _sz, _vp, _vp]),
     ("pk_populate_place", _i, [_vp, _vp, _d, _i, _d, _d, C.c_uint64, _vp, _sz, _vp, _i64, _vp]),
     ("pk_timing_enable", _i, [_i]),
+    ("pk_timing_select", _i, [_i]),
     ("pk_timing_read", _i, [_vp, _vp, _i]),
     ("pk_stage_name", C.c_char_p, [_i]),
 ]

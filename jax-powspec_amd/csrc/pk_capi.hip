@@ -52,6 +52,7 @@ struct Rec {
 };
 std::mutex g_tmu;
 bool g_timing = false;
+int g_timing_only = -1;   // >= 0: only this stage is timed
 std::vector<Rec> g_recs;
 thread_local hipEvent_t g_open[PK_NUM_STAGES] = {};
 const char* g_stage_names[PK_NUM_STAGES] = {"sort_count", "scan",   "sort_place",  "deposit",
@@ -64,7 +65,7 @@ void timing_suspend(bool on);
 bool timing_suspended();
 
 void timing_begin(int stage, hipStream_t s) {
-  if (!g_timing || timing_suspended()) return;
+  if (!g_timing || timing_suspended() || (g_timing_only >= 0 && stage != g_timing_only)) return;
   hipEvent_t e;
   if (hipEventCreate(&e) != hipSuccess) return;
   (void)hipEventRecord(e, s);
@@ -932,6 +933,13 @@ int pk_batch_destroy(pk_batch* b) {
 int pk_timing_enable(int on) {
   std::lock_guard<std::mutex> lk(g_tmu);
   g_timing = on != 0;
+  return PK_OK;
+}
+
+int pk_timing_select(int stage) {
+  if (stage < -1 || stage >= PK_NUM_STAGES) return set_error(PK_E_INVALID, "stage %d not in -1..%d", stage, PK_NUM_STAGES - 1);
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_timing_only = stage;
   return PK_OK;
 }
 
