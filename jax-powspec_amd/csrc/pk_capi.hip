@@ -193,7 +193,7 @@ int make_args(ScatterArgs* a, const float* x, int64_t sx, const float* y, int64_
 size_t powspec_ws_bytes(int n, int nb) {
   const int G = bin_grid(n, nb);
   return align_up((size_t)G * (size_t)(nb > 0 ? nb : 1) * BIN_PART_LEN * sizeof(double)) +
-         align_up((size_t)zpass_grid(n) * sizeof(double));
+         align_up((size_t)2 * zpass_grid(n) * sizeof(double));   // z pass: DC and squares partials
 }
 
 int check_bins(const int32_t* thr, const int32_t* jstart, const int64_t* modes, const double* ksum, int nb,
@@ -210,6 +210,7 @@ void set_bin_fields(Job& j, const pk_plan* p, float2* spec, const int32_t* thr, 
                     const int64_t* modes, const double* ksum, int nb, int window_p, int flags, double box,
                     double shot, double* sums, int64_t* counts, float* pk, float* nmodes, double* kmean) {
   const int n = p->n;
+  j.nm = n;
   j.spec = spec;
   j.wtab = p->wtab + (size_t)window_p * n;
   j.thr = thr;
@@ -260,10 +261,13 @@ int run_powspec(pk_plan* p, const float* mesh, const double* stats, const int32_
   j.partials = static_cast<double*>(ws);
   // with the contrast applied (ZERO_DC) the k = 0 mode is exactly 0 and no DC sum is needed
   const bool keep_dc = (flags & PK_FLAG_ZERO_DC) == 0;
-  double* dc_part = keep_dc ? dc_partials(ws, n, nb) : nullptr;
+  double* zp = dc_partials(ws, n, nb);   // [Gz] DC sums, then [Gz] sums of squares
+  double* dc_part = keep_dc ? zp : nullptr;
   j.dc_part = dc_part;
   j.Gz = zpass_grid(n);
-  int e = fft_zpass(n, mesh, spec, p->tw, stats, dc_part, s);
+  j.sq_part = zp + j.Gz;
+  j.sq_n = j.Gz;
+  int e = fft_zpass(n, mesh, spec, p->tw, stats, dc_part, zp + j.Gz, s);
   if (e) return e;
   return finish_powspec(p, j, s);
 }
@@ -352,7 +356,7 @@ int pk_r2c(pk_plan* p, const float* mesh, const double* stats, void* spectrum, v
   if (!mesh || !spectrum) return set_error(PK_E_INVALID, "null mesh / spectrum");
   hipStream_t s = as_stream(stream);
   float2* spec = static_cast<float2*>(spectrum);
-  e = fft_zpass(p->n, mesh, spec, p->tw, stats, nullptr, s);
+  e = fft_zpass(p->n, mesh, spec, p->tw, stats, nullptr, nullptr, s);
   Job j{};
   j.spec = spec;
   if (!e) e = fft_ypass(p->n, j, p->tw, s);
@@ -499,6 +503,8 @@ int pk_estimate(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t 
                  shot_noise, sums, counts, pk, nmodes, kmean);
   j.partials = static_cast<double*>(ws.powspec_ws);
   j.scale_stats = j.stats;
+  j.sq_part = j.aux + 2;   // the mass finalize sums the tiles' squares there
+  j.sq_n = 1;
   e = fused_deposit_zpass(p->n, j, p->tw, s);
   if (e) return e;
   e = mass_finalize(j, s);
@@ -816,6 +822,8 @@ int pk_batch_set_jobs(pk_batch* b, int njobs, const pk_job* jobs, void* stream) 
     j.dc_part = nullptr;
     j.Gz = 0;
     j.scale_stats = j.stats;
+    j.sq_part = j.aux + 2;
+    j.sq_n = 1;
     h[i] = j;
     if (j.g.nblk > d.max_nblk) d.max_nblk = j.g.nblk;
     if (j.g.nkeys > d.max_nkeys) d.max_nkeys = j.g.nkeys;

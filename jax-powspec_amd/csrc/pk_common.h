@@ -281,6 +281,25 @@ __device__ __forceinline__ double block_sum_f64_waves(double v, double* red) {
   return red[NW];
 }
 
+// The same for int64 (exact, so any order gives the same result).  red needs NT / 64 + 1
+// entries; every thread must call it; the result is in all threads.
+template <int NT>
+__device__ __forceinline__ long long block_sum_i64_waves(long long v, long long* red) {
+  static_assert(NT % 64 == 0, "whole waves only");
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int w = 0; w < NW; ++w) t += red[w];
+    red[NW] = t;
+  }
+  __syncthreads();
+  return red[NW];
+}
+
 __host__ __device__ constexpr int pow2_ceil(int v) {
   int p = 1;
   while (p < v) p <<= 1;

@@ -44,7 +44,7 @@ template <int N> struct FFTCfg {
 template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTZ) __attribute__((amdgpu_waves_per_eu(FFTCfg<N>::WPEZ)))
 k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2* __restrict__ twg,
-        const double* __restrict__ stats, double* __restrict__ dc_part, int kz_lim) {
+        const double* __restrict__ stats, double* __restrict__ dc_part, double* __restrict__ sq_part, int kz_lim) {
   using Cf = FFTCfg<N>;
   constexpr int NC = Cf::NC, C = Cf::CZ, NT = Cf::NTZ, CS = NC + 1;
   constexpr int YT = N / C;
@@ -55,6 +55,7 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   double dc = 0.0;  // FP64 sum of the transformed values = the k = 0 mode
+  double sq = 0.0;  // and of their squares (Parseval bound of the binned power)
   float s = 1.0f, b = 0.0f;
   if (stats != nullptr) {  // contrast delta = rho / mean - 1 (tests/lognormal.py:79-80)
     s = (float)stats[1];
@@ -79,6 +80,7 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
           v[k].x = v[k].x * s + b;
           v[k].y = v[k].y * s + b;
           dc += (double)v[k].x + (double)v[k].y;
+          sq += (double)v[k].x * v[k].x + (double)v[k].y * v[k].y;
           buf[(i / NC) * CS + (i % NC)] = v[k];
         }
       }
@@ -103,6 +105,10 @@ k_zpass(const float* __restrict__ mesh, float2* __restrict__ spec, const float2*
   if (dc_part != nullptr) {  // fixed-order tree: the k = 0 mode in FP64 (sum of all values)
     const double t = block_sum_f64(dc, red, NT);
     if (tid == 0) dc_part[blockIdx.x] = t;
+  }
+  if (sq_part != nullptr) {
+    const double t = block_sum_f64(sq, red, NT);
+    if (tid == 0) sq_part[blockIdx.x] = t;
   }
 }
 
@@ -222,6 +228,31 @@ __device__ __forceinline__ int find_bin(int m, const int* __restrict__ thr, cons
 constexpr int HB_LDS = 256;   // bins held in LDS; above this the per-block slice lives in HBM
 constexpr int NACC = BIN_PART_LEN;   // S0, S2, S4 and the binned mode count per block and bin
 
+// Exponent e of the bins' 64-bit fixed point (a sum is an integer times 2^-e).  Every entry
+// the x pass bins is a stored mode's |X_k|^2 times window factors of at most wmax^3 (wmax:
+// the factor at |k| = n/2, the largest), and the stored modes' |X_k|^2 add up to at most
+// n^3 sum(x^2) over the real field x that was transformed (Parseval).  So 2 wmax^3 n^3
+// sum(x^2) (the 2 covers float rounding) bounds every bin's sums and their total, and e puts
+// that bound at or below 2^62.  The partial sums of x^2 are added in a fixed order, so every
+// block (and the finalize) gets the same e, and integer bins make the sums independent of the
+// order of the atomics.  Every thread must call it (two barriers).
+template <int NT>
+__device__ __forceinline__ int bin_fixed_exp(const Job& j, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < j.sq_n; i += NT) s += j.sq_part[i];
+  const double sq = block_sum_f64_waves<NT>(s, red);
+  const double n = (double)j.nm;
+  const double wmax = (double)j.wtab[j.nm / 2];
+  const double bound = 2.0 * sq * wmax * wmax * wmax * n * n * n;
+  if (!(bound > 0.0) || !(bound < 1e300)) return 0;   // empty field: every sum is 0
+  int ex;
+  (void)frexp(bound, &ex);   // bound < 2^ex
+  return 62 - ex;
+}
+__device__ __forceinline__ unsigned long long to_bin_fixed(float v, double scale) {
+  return (unsigned long long)(long long)rint((double)v * scale);
+}
+
 // ------------------------------------------------------------------------------------
 // x pass fused with deconvolution, |delta_k|^2, Legendre weights and per-bin FP64 sums.
 // Each persistent block accumulates its own histogram and writes it to
@@ -259,7 +290,8 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   constexpr int JLEN = XBinLds<N>::JLEN;
   __shared__ float2 buf[N * PS];
   __shared__ int thr_s[LH ? HB_LDS + 1 : 1];
-  __shared__ double hs[LH ? 3 * HB_LDS : 1];
+  __shared__ unsigned long long hs[LH ? 3 * HB_LDS : 1];   // fixed-point S_0, S_2, S_4
+  __shared__ double red_e[NT / 64 + 1];
   __shared__ unsigned hc[LH ? HB_LDS : 1];   // stored modes binned per bin
   __shared__ float wt_s[LH ? H + 1 : 1];
   __shared__ short jst_s[LH ? JLEN : 1];   // bins < HB_LDS
@@ -270,14 +302,15 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   double* gpart = j.partials + (size_t)bx * nb * NACC;
   if constexpr (LH) {
     for (int i = tid; i <= nb; i += NT) thr_s[i] = thr_g[i];
-    for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0.0;
+    for (int i = tid; i < 3 * HB_LDS; i += NT) hs[i] = 0ull;
     for (int i = tid; i < HB_LDS; i += NT) hc[i] = 0u;
     for (int i = tid; i <= H; i += NT) wt_s[i] = wtab[i];
     for (int i = tid; i < JLEN; i += NT) jst_s[i] = (short)jst_g[i];
   } else {
-    for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;
+    for (int i = tid; i < nb * NACC; i += NT) gpart[i] = 0.0;   // all-zero bits: integer 0 too
   }
   __syncthreads();
+  const double fscale = ldexp(1.0, bin_fixed_exp<NT>(j, red_e));
   // threshold i (0 <= i <= nb) and the raw bin of m: #{i <= nb : thr[i] <= m} - 1, in
   // [-1, nb] (-1: below the first edge, nb: at or above the closed last edge)
   auto thr_at = [&](int i) -> int {
@@ -428,17 +461,20 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     int cnt = 0, cur = -1;
     auto flush = [&]() {
       if (cnt == 0) return;
+      const unsigned long long f0 = to_bin_fixed(a0, fscale);
+      const unsigned long long f2 = to_bin_fixed(a2, fscale);
+      const unsigned long long f4 = to_bin_fixed(a4, fscale);
       if constexpr (LH) {
-        atomicAdd(&hs[cur], (double)a0);
-        atomicAdd(&hs[HB_LDS + cur], (double)a2);
-        atomicAdd(&hs[2 * HB_LDS + cur], (double)a4);
+        atomicAdd(&hs[cur], f0);
+        atomicAdd(&hs[HB_LDS + cur], f2);
+        atomicAdd(&hs[2 * HB_LDS + cur], f4);
         atomicAdd(&hc[cur], (unsigned)cnt);
       } else {
-        double* g = gpart + (size_t)cur * NACC;
-        atomicAdd(g + 0, (double)a0);
-        atomicAdd(g + 1, (double)a2);
-        atomicAdd(g + 2, (double)a4);
-        atomicAdd(g + 3, (double)cnt);   // exact: integers below 2^53
+        unsigned long long* g = reinterpret_cast<unsigned long long*>(gpart) + (size_t)cur * NACC;
+        atomicAdd(g + 0, f0);
+        atomicAdd(g + 1, f2);
+        atomicAdd(g + 2, f4);
+        atomicAdd(g + 3, (unsigned long long)cnt);
       }
     };
     // Each row reads its entries unconditionally (rows clamped to H, columns B = A for
@@ -494,13 +530,13 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     flush();
     __syncthreads();
   }
-  if constexpr (LH) {
+  if constexpr (LH) {   // the block's integer partials (as int64 words)
     for (int i = tid; i < nb; i += NT) {
-      double* g = gpart + (size_t)i * NACC;
+      unsigned long long* g = reinterpret_cast<unsigned long long*>(gpart) + (size_t)i * NACC;
       g[0] = hs[i];
       g[1] = hs[HB_LDS + i];
       g[2] = hs[2 * HB_LDS + i];
-      g[3] = (double)hc[i];
+      g[3] = (unsigned long long)hc[i];
     }
   }
 }
@@ -538,15 +574,22 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   float* __restrict__ pk = j.pk;
   float* __restrict__ nmodes = j.nmodes;
   double* __restrict__ kmean = (j.flags & PK_FLAG_KMEAN) ? j.kmean : nullptr;
-  double part[NACC] = {0.0, 0.0, 0.0, 0.0};
+  __shared__ long long redi[FIN_NT / 64 + 1];
+  // integer partials of the x pass blocks (int64 words), summed exactly, then scaled by the
+  // x pass's fixed-point exponent (recomputed from the same partials in the same order)
+  const double unit = ldexp(1.0, -bin_fixed_exp<FIN_NT>(j, red));
+  long long part[NACC] = {0, 0, 0, 0};
   for (int g = threadIdx.x; g < G; g += FIN_NT) {
-    const double* p = partials + ((size_t)g * nb + b) * NACC;
+    const long long* p = reinterpret_cast<const long long*>(partials) + ((size_t)g * nb + b) * NACC;
 #pragma unroll
     for (int a = 0; a < NACC; ++a) part[a] += p[a];
   }
   double acc[NACC];
 #pragma unroll
-  for (int a = 0; a < NACC; ++a) acc[a] = block_sum_f64_waves<FIN_NT>(part[a], red);
+  for (int a = 0; a < NACC; ++a) {
+    const long long t = block_sum_i64_waves<FIN_NT>(part[a], redi);
+    acc[a] = a < 3 ? (double)t * unit : (double)t;
+  }
   double dcp = 0.0;
   for (int g = threadIdx.x; dc_part != nullptr && g < Gz; g += FIN_NT) dcp += dc_part[g];
   const double dc = dc_part != nullptr ? block_sum_f64_waves<FIN_NT>(dcp, red) : 0.0;
@@ -1217,10 +1260,11 @@ template <int N> struct FFTLaunch {
     return grid;
   }
   static int zpass(const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
-                   hipStream_t s, int kz_lim = N / 2) {
+                   double* sq_part, hipStream_t s, int kz_lim = N / 2) {
     const int grid = zgrid();
     StageTimer t(ST_ZPASS, s);
-    hipLaunchKernelGGL(k_zpass<N>, dim3(grid), dim3(Cf::NTZ), 0, s, mesh, spec, tw, stats, dc_part, kz_lim);
+    hipLaunchKernelGGL(k_zpass<N>, dim3(grid), dim3(Cf::NTZ), 0, s, mesh, spec, tw, stats, dc_part, sq_part,
+                       kz_lim);
     return check_hip(hipGetLastError(), "z pass launch");
   }
   static int ypass(float2* spec, const float2* tw, hipStream_t s) {
@@ -1337,7 +1381,7 @@ template <int N> struct FFTLaunch {
   static int r2c_into(const float* mesh, float2* out, const float2* tw, hipStream_t s, int kz_lim = N / 2) {
     static int gy = 0;
     if (!gy) gy = persistent_grid(k_ypass_lim<N>, Cf::NTY, YTILES);
-    int e = zpass(mesh, out, tw, nullptr, nullptr, s, kz_lim);
+    int e = zpass(mesh, out, tw, nullptr, nullptr, nullptr, s, kz_lim);
     if (e) return e;
     if (kz_lim >= 0) hipLaunchKernelGGL(k_ypass_lim<N>, dim3(gy), dim3(Cf::NTY), 0, s, out, tw, kz_lim);
     e = check_hip(hipGetLastError(), "y pass launch");
@@ -1419,7 +1463,7 @@ template <int N> struct FFTLaunch {
     static int gy = 0, gz = 0;
     if (!gy) gy = persistent_grid(k_ypass_inverse<N>, Cf::NTY, YTILES);
     if (!gz) gz = persistent_grid(k_zpass_c2r_bin<N>, Cf::NTZ, ZTILES);   // sized without dynamic LDS
-    int e = zpass(mesh, spec, tw, nullptr, nullptr, s);
+    int e = zpass(mesh, spec, tw, nullptr, nullptr, nullptr, s);
     if (!e) e = ypass(spec, tw, s);
     if (e) return e;
     hipLaunchKernelGGL(k_xpass_power_inverse<N>, dim3(xi_gx()), dim3(Cf::NTX), 0, s, spec, tw, wt, a.xi0_part);
@@ -1471,9 +1515,9 @@ int ypass_grid(int n) {
 }
 
 int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
-              hipStream_t s) {
+              double* sq_part, hipStream_t s) {
   switch (n) {
-#define PK_CASE(V) case V: return FFTLaunch<V>::zpass(mesh, spec, tw, stats, dc_part, s);
+#define PK_CASE(V) case V: return FFTLaunch<V>::zpass(mesh, spec, tw, stats, dc_part, sq_part, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }

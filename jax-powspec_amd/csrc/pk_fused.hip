@@ -74,6 +74,7 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
   __shared__ float2 tw[N];   // full table: the split and the stages read one entry each
   __shared__ TileBuckets tb[2];
   __shared__ unsigned long long tsum[2];   // exact integer mass of the last two tiles
+  __shared__ double wsq[NT / 64];           // per-wave sums of the tile's (rho - c)^2
   const ScatterArgs a = j.a;
   const TileGeom g = j.g;
   const void* sorted = j.sorted;
@@ -212,6 +213,7 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       constexpr int PER0 = (TASKS + NT - 1) / NT;
       float2 v[PER0][R0];
       long long isum = 0;
+      float sqf = 0.0f;   // (rho - c)^2 of this thread's cells, in a fixed order
 #pragma unroll
       for (int u = 0; u < PER0; ++u) {
         const int task = tid + u * NT;
@@ -241,13 +243,26 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
               v[u][r] = make_float2((float)((double)q[2 * r] * inv_scale) - c0,
                                     (float)((double)q[2 * r + 1] * inv_scale) - c0);
           }
+#pragma unroll
+          for (int r = 0; r < R0; ++r) sqf = fmaf(v[u][r].x, v[u][r].x, fmaf(v[u][r].y, v[u][r].y, sqf));
         }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) isum += __shfl_xor(isum, off, 64);
       if ((tid & 63) == 0) atomicAdd(&tsum[cur], (unsigned long long)isum);
+      {   // the tile's sum of squares (Parseval bound of the binned power), fixed order
+        double sq = (double)sqf;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+        if ((tid & 63) == 0) wsq[tid >> 6] = sq;
+      }
       if (tn < ntiles) prefetch(tb[cur ^ 1]);
       __syncthreads();   // every cell read before the complex rows overwrite them
+      if (tid == 0 && j.tile_sq != nullptr) {
+        double sq = 0.0;
+        for (int w = 0; w < NT / 64; ++w) sq += wsq[w];
+        j.tile_sq[t] = sq;
+      }
 #pragma unroll
       for (int u = 0; u < PER0; ++u) {
         const int task = tid + u * NT;

@@ -65,6 +65,7 @@ struct Job {
   void* staged;          // records grouped by x plane only (the first placing pass)
   unsigned* koff;        // [nkeys] records placed so far in each key (second pass)
   double* tile_mass;     // [ntiles]
+  double* tile_sq;       // [ntiles] fused tiles' sums of (rho - c)^2 (nullptr: not reduced)
   double* stats;         // [PK_STATS_LEN]
   // spectrum and binning
   float2* spec;                // [n/2+1][n][n]
@@ -78,6 +79,12 @@ struct Job {
   double* partials;            // [G][nb][BIN_PART_LEN] x-pass block partials
   const double* dc_part;       // FP64 k = 0 partials of the z pass (nullable)
   int Gz;                      // number of dc_part entries
+  // FP64 partial sums of the squared real field behind the spectrum, summed in a fixed
+  // order: n^3 times their total bounds the binned power (Parseval), which fixes the
+  // exponent of the bins' 64-bit fixed-point sums
+  const double* sq_part;
+  int sq_n;
+  int nm;                      // mesh side of the spectrum
   const double* scale_stats;   // sums scaled by scale_stats[1]^2 (fused contrast), nullable
   double norm;                 // (box / n^2)^3
   double shot;
@@ -95,6 +102,7 @@ constexpr int BIN_PART_LEN = 4;
 bool fft_supported(int n);
 int zpass_grid(int n);
 int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const double* stats, double* dc_part,
+              double* sq_part,
               hipStream_t s);
 int fft_xpass(int n, float2* spec, const float2* tw, hipStream_t s);
 int bin_grid(int n, int nb);   // x-pass blocks of one estimate

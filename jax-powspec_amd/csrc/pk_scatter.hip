@@ -709,6 +709,12 @@ __device__ __forceinline__ void mass_finalize_body(const Job& j, int ntiles) {
     j.stats[0] = m;
     j.stats[1] = n * n * n / m;
   }
+  if (j.tile_sq != nullptr) {   // fused tiles: the sum of (rho - c)^2 in a fixed order -> aux[2]
+    double q = 0.0;
+    for (int i = threadIdx.x; i < ntiles; i += MASS_NT) q += j.tile_sq[i];
+    const double tq = block_sum_f64_waves<MASS_NT>(q, red);
+    if (threadIdx.x == 0) j.aux[2] = tq;
+  }
 }
 __global__ void __launch_bounds__(MASS_NT) k_mass_finalize(Job j, int ntiles) { mass_finalize_body(j, ntiles); }
 __global__ void __launch_bounds__(MASS_NT) k_mass_finalize_jobs(const Job* __restrict__ jobs) {
@@ -840,6 +846,7 @@ size_t carve_scatter(Job& j, bool direct, void* base) {
     j.sorted = take((size_t)(j.a.np > 0 ? j.a.np : 1) * sizeof(float4));
     j.staged = take((size_t)(j.a.np > 0 ? j.a.np : 1) * sizeof(float4));
     j.tile_mass = static_cast<double*>(take((size_t)g.ntiles * sizeof(double)));
+    j.tile_sq = static_cast<double*>(take((size_t)g.ntiles * sizeof(double)));
   }
   return off;
 }
@@ -882,6 +889,7 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
       if (e) return e;
     }
     j.tile_mass = nullptr;
+    j.tile_sq = nullptr;
     return mass_finalize(j, s);
   }
   {
@@ -952,6 +960,7 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
     int e = check_hip(hipGetLastError(), "irregular deposit launch");
     if (e) return e;
   }
+  j.tile_sq = nullptr;   // mesh tiles: the z pass sums the squares
   return mass_finalize(j, s);
 }
 

@@ -315,6 +315,32 @@ def test_estimate_equals_two_step_path(gpu):
     assert np.nanmax(err) < 2e-5
 
 
+def test_bin_sums_are_bitwise_reproducible(gpu, c2_mock):
+    # the bins sum in 64-bit fixed point (a Parseval bound of the binned power fixes the
+    # exponent), so repeated estimates agree bit for bit: the fused path on the 512^3
+    # lognormal + RSD mock, and the mesh path (powspec_vec) on its assigned mesh, each twice
+    box = C2_BOX
+    kF = 2 * np.pi / box
+    edges = kF * (0.5 + np.arange(C2_N // 2 + 1))
+    x, y, z = c2_mock[:, 0], c2_mock[:, 1], c2_mock[:, 2]
+    runs = [correlations.estimate(x, y, z, box_size=box, n_bins=C2_N, k_edges=edges, full=True) for _ in range(2)]
+    for a, b in [(runs[0], runs[1])]:
+        assert torch.equal(a[3]["sums"], b[3]["sums"])
+        assert torch.equal(a[3]["counts"], b[3]["counts"])
+        assert torch.equal(torch.nan_to_num(a[1]), torch.nan_to_num(b[1]))
+    m = mas.assign(x, y, z, box_size=box, n_bins=C2_N, mas="CIC_REF")
+    d = (m / m.mean() - 1.0).float()
+    del m
+    ps = [correlations.powspec_vec(d, box, edges, full=True) for _ in range(2)]
+    assert torch.equal(ps[0][3]["sums"], ps[1][3]["sums"])
+    assert torch.equal(ps[0][3]["counts"], ps[1][3]["counts"])
+    # the two paths transform different float rounding of the same field: close, not equal
+    s1, s2 = runs[0][3]["sums"].cpu().numpy(), ps[0][3]["sums"].cpu().numpy()
+    scale = np.abs(s2).max(axis=0, keepdims=True)
+    print(f"[bitwise] fused vs mesh path: max |diff| / column scale {np.max(np.abs(s1 - s2) / scale):.2e}",
+          flush=True)
+
+
 C2_N, C2_BOX, C2_NPART = 512, 1500.0, 20_000_000
 
 
