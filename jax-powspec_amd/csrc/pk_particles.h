@@ -245,26 +245,25 @@ __device__ __forceinline__ void for_each_corner_on_x(const ScatterArgs& a, float
   }
 }
 
-// The corners on plane xt of a reference-stencil particle whose base cell is inside the
-// mesh (every bucket but the irregular one).  Then every index is valid and the four wrap
-// rules give the same +1 neighbour: 0 at the upper edge, i + 1 + n for i + 1 < 0 (negative
-// positions truncate toward 0 and normalise as JAX indexes), else i + 1; only cic_mas with
-// wrap=False also zeroes the fraction at the upper edge (src/mas.py:34).  Same weights and
-// products, left to right, as for_each_corner_on_x (incl. the :149 corner), without its
-// per-corner validity tests.
+// The corners on plane xt of a reference-stencil particle whose truncated base index is in
+// [0, n) on every axis (every bucket but the irregular one).  Then every index is valid and
+// the four wrap rules give the same +1 neighbour: 0 at the upper edge, else i + 1; only
+// cic_mas with wrap=False also zeroes the fraction at the upper edge (src/mas.py:34).  Same
+// weights and products, left to right, as for_each_corner_on_x (incl. the :149 corner),
+// without its per-corner validity tests.
 template <typename F>
 __device__ __forceinline__ void ref_corners_on_x_regular(const ScatterArgs& a, float px, float py, float pz,
                                                          float w, int xt, F&& f) {
   const int n = a.n;
   const bool scan_false = a.wrap_mode == PK_WRAP_SCAN_FALSE;
   auto axis = [&](float pos, int& b, int& p, float& md, float& dd) {
-    const int i = (int)pos;   // truncation toward zero
+    const int i = (int)pos;   // truncation toward zero; 0 <= i < n here
     const float d = pos - (float)i;
     md = 1.0f - d;
-    const int ip = i + 1;
-    dd = (scan_false && ip >= n) ? 0.0f : d;
-    b = i < 0 ? i + n : i;
-    p = ip >= n ? 0 : (ip < 0 ? ip + n : ip);
+    const bool top = i + 1 >= n;
+    dd = (scan_false && top) ? 0.0f : d;
+    b = i;
+    p = top ? 0 : i + 1;
   };
   int xb, xp, yb, yp, zb, zp;
   float xm, xd, ym, yd, zm, zd;
@@ -325,9 +324,10 @@ __device__ __forceinline__ int row_tile_key(const ScatterArgs& a, const TileGeom
   if (!pos_ok(px, py, pz)) return -1;
   const int n = a.n;
   if (a.stencil == PK_STENCIL_CIC_REF) {
+    // irregular: a truncated base index outside [0, n) on any axis (positions below -1 cell
+    // or at or above n), whose corners need the per-corner JAX index rules
     const int ix = (int)px, iy = (int)py, iz = (int)pz;
-    const int bx = ix < 0 ? ix + n : ix, by = iy < 0 ? iy + n : iy, bz = iz < 0 ? iz + n : iz;
-    if (bx < 0 || bx >= n || by < 0 || by >= n || bz < 0 || bz >= n) {
+    if ((unsigned)ix >= (unsigned)n || (unsigned)iy >= (unsigned)n || (unsigned)iz >= (unsigned)n) {
       if (plane) *plane = n;
       return g.nkeys - 1;
     }
