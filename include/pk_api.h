@@ -158,7 +158,10 @@ int pk_powspec_mesh_noalloc(pk_plan* plan, const float* mesh, const double* stat
 
 /* --- whole estimate -----------------------------------------------------------------
  * particles -> assignment (mesh_in = zeros) -> contrast folded into the FFT read ->
- * P_l(k).  Workspace holds the mesh, the spectrum and all scratch.
+ * P_l(k).  Workspace holds the mesh, the spectrum and all scratch.  Results are bitwise
+ * reproducible: the deposit and the bin sums add integers (64-bit fixed point), so neither
+ * depends on the order of atomics, and batched estimates of the same catalogue equal
+ * pk_estimate's bit for bit.
  */
 int pk_estimate_workspace_size(pk_plan* plan, int64_t np, int stencil, const int* tile, int nb,
                                size_t* bytes);
