@@ -318,12 +318,15 @@ __device__ __forceinline__ float4 load_record(const void* sorted, size_t i, bool
 // into the neighbouring tile (CIC variants: 1 = last row; TSC: 1 = first, 2 = last row).
 // The x neighbours are read whole (one x-layer per tile).  Last key: irregular particles
 // (reference stencil with an out-of-range base index); -1: dropped.
-// plane (optional): the key's x plane hx, or n for the irregular key (keys are x-plane major)
+// plane (optional): the key's x plane hx, or n for the irregular key (keys are x-plane major).
+// ST >= 0 fixes the stencil at compile time (the run-time a.stencil is then ignored).
+template <int ST = -1>
 __device__ __forceinline__ int row_tile_key(const ScatterArgs& a, const TileGeom& g, float px, float py, float pz,
                                             int* plane = nullptr) {
   if (!pos_ok(px, py, pz)) return -1;
   const int n = a.n;
-  if (a.stencil == PK_STENCIL_CIC_REF) {
+  const int stencil = ST >= 0 ? ST : a.stencil;
+  if (stencil == PK_STENCIL_CIC_REF) {
     // irregular: a truncated base index outside [0, n) on any axis (positions below -1 cell
     // or at or above n), whose corners need the per-corner JAX index rules
     const int ix = (int)px, iy = (int)py, iz = (int)pz;
@@ -332,12 +335,12 @@ __device__ __forceinline__ int row_tile_key(const ScatterArgs& a, const TileGeom
       return g.nkeys - 1;
     }
   }
-  const int hx = home_cell(px, n, a.stencil);
-  const int hy = home_cell(py, n, a.stencil);
+  const int hx = home_cell(px, n, stencil);
+  const int hy = home_cell(py, n, stencil);
   const int yt = (int)__umulhi((unsigned)hy, g.ty_mul);   // hy / ty
   const int ly = hy - yt * g.ty;
   int cls;
-  if (a.stencil == PK_STENCIL_TSC) cls = (ly == 0) ? 1 : ((ly == g.ty - 1) ? 2 : 0);
+  if (stencil == PK_STENCIL_TSC) cls = (ly == 0) ? 1 : ((ly == g.ty - 1) ? 2 : 0);
   else cls = (ly == g.ty - 1) ? 1 : 0;
   if (plane) *plane = hx;
   return (hx * g.nyt + yt) * g.ncls + cls;

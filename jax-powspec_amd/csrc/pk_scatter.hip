@@ -65,7 +65,7 @@ struct CountShared {
 
 // AOS / W: the job's input layout and weights, fixed per instantiation (no branch between the
 // loads of a thread's particles)
-template <int AOS, int W>
+template <int AOS, int W, int ST = -1>
 __device__ __forceinline__ void bucket_count_body(const Job& j, int bx, CountShared& sh) {
   // chunks of <= 65535 particles: key k -> 16-bit half (k & 1) of word k >> 1 (half the
   // LDS); larger chunks: one 32-bit counter per key
@@ -105,7 +105,7 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx, CountSha
 #pragma unroll
     for (int u = 0; u < CNT_UNROLL; ++u) {
       if (i0 + (int64_t)u * CNT_NT >= hi) continue;
-      const int key = row_tile_key(a, g, px[u], py[u], pz[u]);
+      const int key = row_tile_key<ST>(a, g, px[u], py[u], pz[u]);
       if (key >= 0) {
         if (g.gtab) atomicAdd(&row[key], 1u);
         else if (packed) atomicAdd(&hist[key >> 1], 1u << (16 * (key & 1)));
@@ -123,10 +123,10 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx, CountSha
   const double t = block_sum_f64(ws, red, CNT_NT);
   if (tid == 0) j.wsum[bx] = t;
 }
-template <int AOS, int W>
+template <int AOS, int W, int ST>
 __global__ void __launch_bounds__(CNT_NT) k_bucket_count(Job j) {
   __shared__ CountShared sh;
-  bucket_count_body<AOS, W>(j, blockIdx.x, sh);
+  bucket_count_body<AOS, W, ST>(j, blockIdx.x, sh);
 }
 __global__ void __launch_bounds__(CNT_NT) k_bucket_count_jobs(const Job* __restrict__ jobs) {
   __shared__ CountShared sh;
@@ -383,7 +383,7 @@ __host__ __device__ constexpr int place_round(int n) {
 }
 size_t place_lds(int n) { return 12 * ((size_t)n + 1) + 20 * (size_t)place_round(n); }
 
-template <int AOS, int W>
+template <int AOS, int W, int ST = -1>
 __device__ __forceinline__ void bucket_place_body(const Job& j, int bx, unsigned* wtot) {
   extern __shared__ unsigned lds[];
   const ScatterArgs a = j.a;
@@ -437,7 +437,7 @@ __device__ __forceinline__ void bucket_place_body(const Job& j, int bx, unsigned
       pl[u] = -1;
       if (u * CNT_NT < S && i < hi) {
         int plane;
-        if (row_tile_key(a, g, px[u], py[u], pz[u], &plane) >= 0) pl[u] = plane;
+        if (row_tile_key<ST>(a, g, px[u], py[u], pz[u], &plane) >= 0) pl[u] = plane;
       }
     }
     float nx[PLACE_PER], ny[PLACE_PER], nz[PLACE_PER], nw[PLACE_PER];
@@ -480,10 +480,10 @@ __device__ __forceinline__ void bucket_place_body(const Job& j, int bx, unsigned
     __syncthreads();   // the staging is read before the next round overwrites it
   }
 }
-template <int AOS, int W>
+template <int AOS, int W, int ST>
 __global__ void __launch_bounds__(CNT_NT) k_bucket_place(Job j) {
   __shared__ unsigned wtot[CNT_NT / 64];
-  bucket_place_body<AOS, W>(j, blockIdx.x, wtot);
+  bucket_place_body<AOS, W, ST>(j, blockIdx.x, wtot);
 }
 __global__ void __launch_bounds__(CNT_NT) k_bucket_place_jobs(const Job* __restrict__ jobs) {
   __shared__ unsigned wtot[CNT_NT / 64];
@@ -498,15 +498,21 @@ __global__ void __launch_bounds__(CNT_NT) k_bucket_place_jobs(const Job* __restr
   }
 }
 
-// the plain launches' instantiation for a job's layout and weights
-template <template <int, int> class Pick>
-auto pick_aw(const ScatterArgs& a) -> decltype(Pick<0, 0>::k) {
-  const bool w = a.w != nullptr;
-  if (a.aos) return w ? Pick<1, 1>::k : Pick<1, 0>::k;
-  return w ? Pick<0, 1>::k : Pick<0, 0>::k;
+// the plain launches' instantiation for a job's layout, weights and stencil
+template <template <int, int, int> class Pick, int AOS, int W>
+auto pick_st(const ScatterArgs& a) -> decltype(Pick<0, 0, 0>::k) {
+  if (a.stencil == PK_STENCIL_CIC_REF) return Pick<AOS, W, PK_STENCIL_CIC_REF>::k;
+  if (a.stencil == PK_STENCIL_CIC) return Pick<AOS, W, PK_STENCIL_CIC>::k;
+  return Pick<AOS, W, PK_STENCIL_TSC>::k;
 }
-template <int AOS, int W> struct CountK { static constexpr void (*k)(Job) = k_bucket_count<AOS, W>; };
-template <int AOS, int W> struct PlaceK { static constexpr void (*k)(Job) = k_bucket_place<AOS, W>; };
+template <template <int, int, int> class Pick>
+auto pick_aw(const ScatterArgs& a) -> decltype(Pick<0, 0, 0>::k) {
+  const bool w = a.w != nullptr;
+  if (a.aos) return w ? pick_st<Pick, 1, 1>(a) : pick_st<Pick, 1, 0>(a);
+  return w ? pick_st<Pick, 0, 1>(a) : pick_st<Pick, 0, 0>(a);
+}
+template <int AOS, int W, int ST> struct CountK { static constexpr void (*k)(Job) = k_bucket_count<AOS, W, ST>; };
+template <int AOS, int W, int ST> struct PlaceK { static constexpr void (*k)(Job) = k_bucket_place<AOS, W, ST>; };
 
 // ------------------------------------------------------------------------------------
 // 3b. fine place: a segment of the plane-grouped records -> one run per (segment, key)
@@ -519,6 +525,7 @@ constexpr size_t FINE_LDS = (size_t)FINE_SEG * 20;   // staging: x, y, z, w, des
 
 int fine_place_blocks(int64_t np) { return (int)((np > 0 ? np + FINE_SEG - 1 : FINE_SEG) / FINE_SEG); }
 
+template <int ST = -1>
 __device__ __forceinline__ void fine_place_body(const Job& j, int bx) {
   extern __shared__ unsigned lds[];
   __shared__ unsigned cnt[FINE_WIN];     // per key of the window: count, then local start
@@ -543,7 +550,7 @@ __device__ __forceinline__ void fine_place_body(const Job& j, int bx) {
   if (tid < 2) {
     const float4 r = load_record(j.staged, (size_t)(tid == 0 ? s0 : s1 - 1), has_w);
     int plane = 0;
-    (void)row_tile_key(a, g, r.x, r.y, r.z, &plane);
+    (void)row_tile_key<ST>(a, g, r.x, r.y, r.z, &plane);
     win[tid] = plane;
   }
   float4 r[FINE_PER];
@@ -556,7 +563,7 @@ __device__ __forceinline__ void fine_place_body(const Job& j, int bx) {
   for (int u = 0; u < FINE_PER; ++u) {
     const int64_t i = s0 + tid + (int64_t)u * FINE_NT;
     r[u].w = record_weight(r[u], has_w);
-    key[u] = i < s1 ? row_tile_key(a, g, r[u].x, r[u].y, r[u].z) : -1;   // the key the coarse pass saw
+    key[u] = i < s1 ? row_tile_key<ST>(a, g, r[u].x, r[u].y, r[u].z) : -1;   // the key the coarse pass saw
   }
   __syncthreads();
   const int kbase = win[0] * (int)kpp;
@@ -597,7 +604,8 @@ __device__ __forceinline__ void fine_place_body(const Job& j, int bx) {
   __syncthreads();
   for (int q = tid; q < (int)m; q += FINE_NT) store_record(j.sorted, sd[q], has_w, sx[q], sy[q], sz[q], sw[q]);
 }
-__global__ void __launch_bounds__(FINE_NT) k_fine_place(Job j) { fine_place_body(j, blockIdx.x); }
+template <int ST>
+__global__ void __launch_bounds__(FINE_NT) k_fine_place(Job j) { fine_place_body<ST>(j, blockIdx.x); }
 __global__ void __launch_bounds__(FINE_NT) k_fine_place_jobs(const Job* __restrict__ jobs) {
   const Job& j = jobs[blockIdx.y];
   if (!j.active) return;
@@ -928,9 +936,12 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
   }
   {
     StageTimer t(ST_SORT_FINE, s);
-    int e = allow_lds(k_fine_place, FINE_LDS);
+    const auto kf = a.stencil == PK_STENCIL_CIC_REF ? k_fine_place<PK_STENCIL_CIC_REF>
+                    : a.stencil == PK_STENCIL_CIC   ? k_fine_place<PK_STENCIL_CIC>
+                                                    : k_fine_place<PK_STENCIL_TSC>;
+    int e = allow_lds(kf, FINE_LDS);
     if (e) return e;
-    hipLaunchKernelGGL(k_fine_place, dim3(fine_place_blocks(a.np)), dim3(FINE_NT), FINE_LDS, s, j);
+    hipLaunchKernelGGL(kf, dim3(fine_place_blocks(a.np)), dim3(FINE_NT), FINE_LDS, s, j);
     e = check_hip(hipGetLastError(), "fine place launch");
     if (e) return e;
   }
