@@ -236,15 +236,18 @@ constexpr int NACC = BIN_PART_LEN;   // S0, S2, S4 and the binned mode count per
 // that bound at or below 2^62.  The partial sums of x^2 are added in a fixed order, so every
 // block (and the finalize) gets the same e, and integer bins make the sums independent of the
 // order of the atomics.  Every thread must call it (two barriers).
+// finite (optional): false when the field holds a NaN or an infinity, whose power the
+// finalize then reports as NaN (as FP64 sums of such a field would be).
 template <int NT>
-__device__ __forceinline__ int bin_fixed_exp(const Job& j, double* red) {
+__device__ __forceinline__ int bin_fixed_exp(const Job& j, double* red, bool* finite = nullptr) {
   double s = 0.0;
   for (int i = threadIdx.x; i < j.sq_n; i += NT) s += j.sq_part[i];
   const double sq = block_sum_f64_waves<NT>(s, red);
   const double n = (double)j.nm;
   const double wmax = (double)j.wtab[j.nm / 2];
   const double bound = 2.0 * sq * wmax * wmax * wmax * n * n * n;
-  if (!(bound > 0.0) || !(bound < 1e300)) return 0;   // empty field: every sum is 0
+  if (finite) *finite = bound == bound && bound < 1e300;
+  if (!(bound > 0.0) || !(bound < 1e300)) return 0;   // empty (or non-finite) field
   int ex;
   (void)frexp(bound, &ex);   // bound < 2^ex
   return 62 - ex;
@@ -577,7 +580,8 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
   __shared__ long long redi[FIN_NT / 64 + 1];
   // integer partials of the x pass blocks (int64 words), summed exactly, then scaled by the
   // x pass's fixed-point exponent (recomputed from the same partials in the same order)
-  const double unit = ldexp(1.0, -bin_fixed_exp<FIN_NT>(j, red));
+  bool finite = true;
+  const double unit = ldexp(1.0, -bin_fixed_exp<FIN_NT>(j, red, &finite));
   long long part[NACC] = {0, 0, 0, 0};
   for (int g = threadIdx.x; g < G; g += FIN_NT) {
     const long long* p = reinterpret_cast<const long long*>(partials) + ((size_t)g * nb + b) * NACC;
@@ -588,7 +592,7 @@ __device__ __forceinline__ void bin_finalize_body(const Job& j, int G, int b) {
 #pragma unroll
   for (int a = 0; a < NACC; ++a) {
     const long long t = block_sum_i64_waves<FIN_NT>(part[a], redi);
-    acc[a] = a < 3 ? (double)t * unit : (double)t;
+    acc[a] = a < 3 ? (finite ? (double)t * unit : __builtin_nan("")) : (double)t;
   }
   double dcp = 0.0;
   for (int g = threadIdx.x; dc_part != nullptr && g < Gz; g += FIN_NT) dcp += dc_part[g];

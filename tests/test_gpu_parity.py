@@ -341,6 +341,25 @@ def test_bin_sums_are_bitwise_reproducible(gpu, c2_mock):
           flush=True)
 
 
+def test_non_finite_and_zero_fields(gpu):
+    # a NaN or an infinity anywhere in the field makes every bin's power NaN (as FP64 sums of
+    # it would be); an all-zero field has zero power in every bin (no fixed-point scale)
+    n, box = 32, 200.0
+    edges = (2 * np.pi / box) * (0.5 + np.arange(n // 2 + 1))
+    rng = np.random.default_rng(5)
+    base = torch.as_tensor(rng.standard_normal((n, n, n)).astype(np.float32), device=gpu)
+    for bad in (float("nan"), float("inf")):
+        d = base.clone()
+        d[3, 4, 5] = bad
+        k, pk, nm, ex = correlations.powspec_vec(d, box, edges, full=True)
+        assert torch.isnan(pk).all(), bad
+        assert torch.isnan(ex["sums"]).all(), bad
+        assert torch.equal(ex["counts"], correlations.powspec_vec(base, box, edges, full=True)[3]["counts"])
+    k, pk, nm, ex = correlations.powspec_vec(torch.zeros((n, n, n), device=gpu), box, edges, full=True)
+    assert torch.equal(ex["sums"], torch.zeros_like(ex["sums"]))
+    assert torch.isfinite(pk).all()
+
+
 C2_N, C2_BOX, C2_NPART = 512, 1500.0, 20_000_000
 
 
