@@ -285,7 +285,6 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
   using Cf = FFTCfg<N>;
   constexpr int C = Cf::CX, NT = Cf::NTX, PS = C + 1;
   constexpr int YT = N / C;
-  constexpr int NTILES = (N / 2 + 1) * YT;
   constexpr int H = N / 2;
   // LDS: the tile, the S_0,2,4 histogram, the thresholds, the window factors of |k| <= N/2,
   // the isqrt-start table and the first half of the twiddles (<= 80 KB at N = 512 so two
@@ -413,20 +412,102 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     if constexpr (S0REG) store_stage0(v);
     else store_tile(v);
   };
+  // Work items enumerate only the tiles that hold a binned mode.  A tile whose columns all
+  // have min ky^2 + kz^2 >= thr[nb] (the end of the last bin) holds only modes outside every
+  // bin; those are neither loaded nor transformed, and the remaining tiles are dealt out to
+  // the blocks evenly.  Plane kz keeps the tiles yt < lo(kz) and yt >= YT - hi(kz): the
+  // smallest |ky| of a tile grows with yt up to the middle and falls beyond it (QUAD: strip
+  // A starts at |ky| = HW yt and strip B's |ky| exceeds it, so hi = 0).  The table of the
+  // planes' first work items and lo(kz) sits in the padding column c = C of buf, which no
+  // tile store, transform stage or binning read touches.
   // Strips narrower than a 128-byte line (QUAD with HW < 16): the blocks of work items t and
   // t + 8 run on the same XCD at about the same time, so they take the two strips of each
-  // line (tiles 2P and 2P + 1) and the second read finds the line in L2.
-  auto tile_of = [](int t) -> int {
-    if constexpr (QUAD && HW * 8 < 128 && YT % 2 == 0) {
-      if (t < (NTILES / 16) * 16) return 2 * ((t / 16) * 8 + (t % 8)) + ((t % 16) / 8);
+  // line (tiles 2P and 2P + 1: lo(kz) is rounded up to even) and the second read finds the
+  // line in L2.
+  constexpr bool PAIRS = QUAD && HW * 8 < 128 && YT % 2 == 0;
+  constexpr int KZN = H + 1;                      // planes kz = 0..H
+  constexpr int KPT = (KZN + NT - 1) / NT;        // planes per lane
+  constexpr int NW = NT / 64;
+  static_assert(KZN + 1 + NW <= N, "plane table fits the padding column");
+  int2* const ktab = reinterpret_cast<int2*>(buf);
+  auto kslot = [](int i) -> int { return i * PS + C; };
+  {
+    const int thr_end = thr_at(nb);
+    // smallest |ky| over the columns of tile yt
+    auto kmin_of = [](int ytt) -> int {
+      if constexpr (QUAD) {
+        return HW * ytt;
+      } else {
+        const int y0 = ytt * C, y1 = y0 + C - 1;
+        return min(y0 <= H ? y0 : N, y1 > H ? N - y1 : N);
+      }
+    };
+    int lov[KPT], cntv[KPT], tsum = 0;
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int kz = tid * KPT + k;
+      int lo = 0, hi = 0;
+      if (kz < KZN) {
+        const int rem = thr_end - kz * kz;   // needed: kmin^2 < rem
+        while (lo < YT && kmin_of(lo) * kmin_of(lo) < rem) ++lo;
+        if constexpr (!QUAD) {
+          while (hi < YT - lo && kmin_of(YT - 1 - hi) * kmin_of(YT - 1 - hi) < rem) ++hi;
+        }
+        if constexpr (PAIRS) lo += lo & 1;
+      }
+      lov[k] = lo;
+      cntv[k] = lo + hi;
+      tsum += lo + hi;
     }
-    return t;
-  };
-  if constexpr (XPF) {
-    if (bx < NTILES) load_tile(tile_of(bx), xpre);
+    const int lane = tid & 63, wid = tid >> 6;
+    int incl = tsum;
+#pragma unroll
+    for (int o = 1; o < 64; o *= 2) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (lane == 63) ktab[kslot(KZN + 1 + wid)].x = incl;
+    __syncthreads();
+    int run = incl - tsum, total = 0;
+    for (int w = 0; w < NW; ++w) {
+      const int wt_w = ktab[kslot(KZN + 1 + w)].x;
+      run += w < wid ? wt_w : 0;
+      total += wt_w;
+    }
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int kz = tid * KPT + k;
+      if (kz < KZN) ktab[kslot(kz)] = make_int2(run, lov[k]);
+      run += cntv[k];
+    }
+    if (tid == 0) ktab[kslot(KZN)] = make_int2(total, 0);
+    __syncthreads();
   }
-  for (int item = bx; item < NTILES; item += nbx) {
-    const int tile = tile_of(item);
+  const int NC = ktab[kslot(KZN)].x;   // work items
+  // tile of work item t (block-uniform): the plane is the last one whose first item is <= c
+  auto tile_of = [&](int t) -> int {
+    int c = t;
+    if constexpr (PAIRS) {
+      if (t < (NC / 16) * 16) c = 2 * ((t / 16) * 8 + (t % 8)) + ((t % 16) / 8);
+    }
+    int lo_i = 0, hi_i = KZN;   // first(lo_i) <= c < first(hi_i)
+    while (hi_i - lo_i > 1) {
+      const int mid = (lo_i + hi_i) >> 1;
+      if (ktab[kslot(mid)].x <= c) lo_i = mid;
+      else hi_i = mid;
+    }
+    const int2 e = ktab[kslot(lo_i)];
+    const int jj = c - e.x;
+    return lo_i * YT + (jj < e.y ? jj : YT - 1 - (jj - e.y));
+  };
+  int item = bx;
+  int tile = item < NC ? tile_of(item) : 0;
+  if constexpr (XPF) {
+    if (item < NC) load_tile(tile, xpre);
+  }
+  while (item < NC) {
+    const int next = item + nbx;
+    const int tile_next = next < NC ? tile_of(next) : 0;
     const int kz = tile / YT, yt = tile % YT;
     const int kz2 = kz * kz;
     // this lane's task: |ky| (tile columns below, after the transform)
@@ -452,7 +533,7 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     }
     __syncthreads();
     if constexpr (XPF) {
-      if (item + nbx < NTILES) load_tile(tile_of(item + nbx), xpre);
+      if (next < NC) load_tile(tile_next, xpre);
     }
     if constexpr (S0REG) FFTStages<N, 1, R0, C, PS, 1, NT, true, 1, 0>::run(buf, tw);
     else lds_fft<N, C, PS, 1, NT, true, 1>(buf, tw);
@@ -532,6 +613,8 @@ __device__ __forceinline__ void xpass_bin_body(const Job& j, const float2* __res
     }
     flush();
     __syncthreads();
+    item = next;
+    tile = tile_next;
   }
   if constexpr (LH) {   // the block's integer partials (as int64 words)
     for (int i = tid; i < nb; i += NT) {
