@@ -146,15 +146,25 @@ __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const floa
       }
     }
   };
-  if (bx < NTILES) load(bx);
-  for (int tile = bx; tile < NTILES; tile += nbx) {
+  // Work item t -> tile.  Whole passes take the rows x-plane major from the highest x down:
+  // the z pass writes the rows x-plane by x-plane upwards, so the first rows read here are
+  // the last it wrote.
+  auto tile_of = [&](int t) -> int {
+    if (ntiles >= 0) return t;
+    constexpr int KZN = N / 2 + 1, XG = N / C;
+    const int u = NTILES - 1 - t;
+    return (u % KZN) * XG + u / KZN;
+  };
+  if (bx < NTILES) load(tile_of(bx));
+  for (int item = bx; item < NTILES; item += nbx) {
+    const int tile = tile_of(item);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = tid + k * NT;
       if (i < C * N) buf[at(i % N, i / N)] = pre[k];
     }
     __syncthreads();
-    if (tile + nbx < NTILES) load(tile + nbx);
+    if (item + nbx < NTILES) load(tile_of(item + nbx));
     lds_fft<N, C, 1, CS, NT, false, 1, SK>(buf, tw);
     float2* row = spec + (size_t)tile * C * N;
     for (int i = tid; i < C * N; i += NT) {
