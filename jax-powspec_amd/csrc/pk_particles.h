@@ -346,6 +346,20 @@ __device__ __forceinline__ int row_tile_key(const ScatterArgs& a, const TileGeom
   return (hx * g.nyt + yt) * g.ncls + cls;
 }
 
+// The x plane of row_tile_key's key (n for the irregular key), or -1 where it returns -1:
+// all the coarse placing pass needs, without the key's y tile and class.
+template <int ST = -1>
+__device__ __forceinline__ int row_plane(const ScatterArgs& a, float px, float py, float pz) {
+  if (!pos_ok(px, py, pz)) return -1;
+  const int n = a.n;
+  const int stencil = ST >= 0 ? ST : a.stencil;
+  if (stencil == PK_STENCIL_CIC_REF) {
+    const int ix = (int)px, iy = (int)py, iz = (int)pz;
+    if ((unsigned)ix >= (unsigned)n || (unsigned)iy >= (unsigned)n || (unsigned)iz >= (unsigned)n) return n;
+  }
+  return home_cell(px, n, stencil);
+}
+
 // Buckets whose particles can deposit into tile (xt, yt): filled into key[] (<= 16).
 __host__ __device__ __forceinline__ int row_tile_buckets(const TileGeom& g, int n, bool tsc, int xt, int yt, int* keys) {
   int xs[3], nxs = 0;

@@ -434,11 +434,7 @@ __device__ __forceinline__ void bucket_place_body(const Job& j, int bx, unsigned
 #pragma unroll
     for (int u = 0; u < PLACE_PER; ++u) {
       const int64_t i = r0 + (int64_t)u * CNT_NT + tid;
-      pl[u] = -1;
-      if (u * CNT_NT < S && i < hi) {
-        int plane;
-        if (row_tile_key<ST>(a, g, px[u], py[u], pz[u], &plane) >= 0) pl[u] = plane;
-      }
+      pl[u] = (u * CNT_NT < S && i < hi) ? row_plane<ST>(a, px[u], py[u], pz[u]) : -1;
     }
     float nx[PLACE_PER], ny[PLACE_PER], nz[PLACE_PER], nw[PLACE_PER];
     load_round(r0 + S, nx, ny, nz, nw);
