@@ -229,10 +229,11 @@ void set_bin_fields(Job& j, const pk_plan* p, float2* spec, const int32_t* thr, 
   j.kmean = kmean;
 }
 
-// y pass, x pass + binning and finalize on a spectrum whose z pass is done
-int finish_powspec(pk_plan* p, const Job& j, hipStream_t s) {
+// y pass, x pass + binning and finalize on a spectrum whose z pass is done (fused: from the
+// fused deposit, whose mass finalize the y pass does)
+int finish_powspec(pk_plan* p, const Job& j, hipStream_t s, bool fused = false) {
   const int n = p->n;
-  int e = fft_ypass(n, j, p->tw, s);
+  int e = fused ? fft_ypass_mass(n, j, p->tw, s) : fft_ypass(n, j, p->tw, s);
   if (e) return e;
   const int G = bin_grid(n, j.nb);
   e = fft_xpass_bin(n, j, p->tw, G, s);
@@ -507,9 +508,7 @@ int pk_estimate(pk_plan* p, const float* x, int64_t sx, const float* y, int64_t 
   j.sq_n = 1;
   e = fused_deposit_zpass(p->n, j, p->tw, s);
   if (e) return e;
-  e = mass_finalize(j, s);
-  if (e) return e;
-  return finish_powspec(p, j, s);
+  return finish_powspec(p, j, s, true);
 }
 
 namespace {
@@ -681,8 +680,6 @@ int batch_launch(pk_batch* b, hipStream_t s) {
   int e = scatter_sort_jobs(b->d_jobs, b->dims, s);
   if (e) return e;
   e = fused_deposit_zpass_jobs(n, b->cap.ty, b->d_jobs, nj, b->cap.ntiles, tw, s);
-  if (e) return e;
-  e = mass_finalize_jobs(b->d_jobs, nj, s);
   if (e) return e;
   int gy = (int)((int64_t)ypass_jobs_grid(n, nj) * b->ypass_pct / 100);
   if (gy < 1) gy = 1;

@@ -176,9 +176,53 @@ __device__ __forceinline__ void ypass_body(float2* __restrict__ spec, const floa
   }
 }
 
+// The mass finalize of a fused estimate, done by one block of NT threads: stats[0] = M (the
+// tiles' exact masses summed in a fixed order, plus stats[2]), stats[1] = n^3 / M, and
+// aux[2] = the tiles' sums of (rho - c)^2 in a fixed order (the binning's fixed-point
+// bound).  U loads per array in flight per thread.
+template <int NT>
+__device__ __forceinline__ void mass_finalize_block(const Job& j) {
+  __shared__ double red[NT / 64 + 1];
+  constexpr int U = 16;
+  const int nt = j.g.ntiles;
+  double m = 0.0, q = 0.0;
+  for (int i0 = threadIdx.x; i0 < nt; i0 += NT * U) {   // block-uniform trip count
+    double vm[U], vq[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = min(i0 + u * NT, nt - 1);
+      vm[u] = j.tile_mass[i];
+      vq[u] = j.tile_sq[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i0 + u * NT < nt) {
+        m += vm[u];
+        q += vq[u];
+      }
+    }
+  }
+  const double tm = block_sum_f64_waves<NT>(m, red) + j.stats[2];
+  const double tq = block_sum_f64_waves<NT>(q, red);
+  if (threadIdx.x == 0) {
+    const double n = (double)j.a.n;
+    j.stats[0] = tm;
+    j.stats[1] = n * n * n / tm;
+    j.aux[2] = tq;
+  }
+}
+
 template <int N>
 __global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass(float2* __restrict__ spec, const float2* __restrict__ twg) {
   ypass_body<N>(spec, twg, blockIdx.x, gridDim.x);
+}
+
+// y pass of a fused estimate whose last block first does the mass finalize (the last block
+// has at most as many tiles as any other)
+template <int N>
+__global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_mass(Job j, const float2* __restrict__ twg) {
+  if (blockIdx.x == gridDim.x - 1) mass_finalize_block<FFTCfg<N>::NTY>(j);
+  ypass_body<N>(j.spec, twg, blockIdx.x, gridDim.x);
 }
 
 template <int N>
@@ -192,6 +236,7 @@ __global__ void __launch_bounds__(FFTCfg<N>::NTY) k_ypass_jobs(const Job* __rest
                                                                const float2* __restrict__ twg) {
   const Job& j = jobs[blockIdx.y];
   if (!j.active) return;
+  if (blockIdx.x == gridDim.x - 1) mass_finalize_block<FFTCfg<N>::NTY>(j);   // fused jobs
   ypass_body<N>(j.spec, twg, blockIdx.x, gridDim.x);
 }
 
@@ -1371,6 +1416,13 @@ template <int N> struct FFTLaunch {
     hipLaunchKernelGGL(k_ypass<N>, dim3(grid), dim3(Cf::NTY), 0, s, spec, tw);
     return check_hip(hipGetLastError(), "y pass launch");
   }
+  static int ypass_mass(const Job& j, const float2* tw, hipStream_t s) {
+    static int grid = 0;
+    if (!grid) grid = persistent_grid(k_ypass_mass<N>, Cf::NTY, YTILES);
+    StageTimer t(ST_YPASS, s);
+    hipLaunchKernelGGL(k_ypass_mass<N>, dim3(grid), dim3(Cf::NTY), 0, s, j, tw);
+    return check_hip(hipGetLastError(), "y pass launch");
+  }
   static int ypass_jobs(const Job* jobs, int njobs, int gy, const float2* tw, hipStream_t s) {
     if (gy < 1) gy = 1;
     if (gy > YTILES) gy = YTILES;
@@ -1625,6 +1677,15 @@ int fft_zpass(int n, const float* mesh, float2* spec, const float2* tw, const do
 int fft_ypass(int n, const Job& j, const float2* tw, hipStream_t s) {
   switch (n) {
 #define PK_CASE(V) case V: return FFTLaunch<V>::ypass(j.spec, tw, s);
+    PK_FOR_EACH_N(PK_CASE)
+#undef PK_CASE
+  }
+  return set_error(PK_E_INVALID, "unsupported n=%d", n);
+}
+
+int fft_ypass_mass(int n, const Job& j, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define PK_CASE(V) case V: return FFTLaunch<V>::ypass_mass(j, tw, s);
     PK_FOR_EACH_N(PK_CASE)
 #undef PK_CASE
   }

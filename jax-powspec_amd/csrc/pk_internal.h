@@ -111,6 +111,10 @@ int ypass_jobs_grid(int n, int njobs);
 int bin_jobs_grid(int n, int nb, int njobs);
 // single job (by value) or a device table of njobs (then gy / gx = blocks per job)
 int fft_ypass(int n, const Job& j, const float2* tw, hipStream_t s);
+// y pass of a fused estimate, whose last block also does the mass finalize (stats[0],
+// stats[1], aux[2] from the tiles' masses and squares); the batched y pass does the same
+// for every job
+int fft_ypass_mass(int n, const Job& j, const float2* tw, hipStream_t s);
 int fft_xpass_bin(int n, const Job& j, const float2* tw, int grid, hipStream_t s);
 int bin_finalize(const Job& j, int G, hipStream_t s);
 // backward of the binning: per-bin coefficients coef[nb][3] (float) and j's forward
@@ -153,7 +157,6 @@ struct BatchDims {
   int max_fblk;   // blocks of the second placing pass of the largest job
 };
 int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s);
-int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s);
 
 // ---- backward of the assignment (pk_adjoint.hip) ------------------------------------------
 // coef[b][l] = grad_pk[b][l] norm (2l+1) / counts[b] (0 for empty bins)

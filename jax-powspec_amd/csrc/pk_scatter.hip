@@ -256,29 +256,12 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_local_jobs(const Job* __restri
   scan_local_body(j, blockIdx.x);
 }
 
-// scans the block sums; sums the weight partials into stats[3] = sum(w) / n^3 and sets
-// the fixed-point scale aux[0] = S, aux[1] = 1 / S from max |w|
-__device__ __forceinline__ void scan_top_body(const Job& j) {
-  __shared__ int sh[SCAN_NT];
+// key_start[nkeys] = the records' total; sums the weight partials into stats[3] =
+// sum(w) / n^3 and sets the fixed-point scale aux[0] = S, aux[1] = 1 / S from max |w|.
+// Every thread of the block must call it.
+__device__ __forceinline__ void scan_finish_body(const Job& j, int total) {
   __shared__ double shd[SCAN_NT];
   __shared__ unsigned long long wmax_bits;
-  const int nb2 = (j.g.nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  int* bsum = j.bsum;
-  const int base = threadIdx.x * SCAN_PER;
-  int v[SCAN_PER];
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k) {
-    v[k] = (base + k < nb2) ? bsum[base + k] : 0;
-    s += v[k];
-  }
-  int total;
-  int run = block_exclusive_scan(s, sh, total);
-#pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k) {
-    if (base + k < nb2) bsum[base + k] = run;
-    run += v[k];
-  }
   if (threadIdx.x == 0) j.key_start[j.g.nkeys] = total;
   double ws = 0.0, wm = 0.0;
   if (threadIdx.x == 0) wmax_bits = 0ull;
@@ -301,27 +284,33 @@ __device__ __forceinline__ void scan_top_body(const Job& j) {
     j.aux[1] = ldexp(1.0, e - 30);
   }
 }
-__global__ void __launch_bounds__(SCAN_NT) k_scan_top(Job j) { scan_top_body(j); }
-// all keys in one local block: the local scan, then the top step (its exclusive scan of
-// the single block sum is 0, so no add pass follows)
+// all keys in one local block: the local scan, then the finish (no add pass follows)
 __global__ void __launch_bounds__(SCAN_NT) k_scan_single(Job j) {
   scan_local_body(j, 0);
   __syncthreads();
-  scan_top_body(j);
-}
-__global__ void __launch_bounds__(SCAN_NT) k_scan_top_jobs(const Job* __restrict__ jobs) {
-  PK_JOB_OR_RETURN(jobs, 1);
-  scan_top_body(j);
+  scan_finish_body(j, j.bsum[0]);
 }
 
+// Block bx adds the sum of the block sums before it to its keys' local starts (each block
+// sums that short prefix itself, so no separate scan of the block sums runs); block 0 also
+// finishes (the records' total and the weight statistics).
 __device__ __forceinline__ void scan_add_body(const Job& j, int bx) {
+  __shared__ int sh[SCAN_NT / 64];
   const int n = j.g.nkeys;
+  const int nb2 = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  const int* bsum = j.bsum;
+  const int upto = bx == 0 ? nb2 : bx;   // block 0: every block sum (the total); else those before bx
+  int s = 0;
+  for (int i = threadIdx.x; i < upto; i += SCAN_NT) s += bsum[i];
+  int sum;
+  (void)block_exclusive_scan(s, sh, sum);
   int* out = j.key_start;
   const int base = bx * SCAN_BLOCK + threadIdx.x * SCAN_PER;
-  const int add = j.bsum[bx];
+  const int add = bx == 0 ? 0 : sum;
 #pragma unroll
   for (int k = 0; k < SCAN_PER; ++k)
     if (base + k < n) out[base + k] += add;
+  if (bx == 0) scan_finish_body(j, sum);   // block-uniform
 }
 __global__ void __launch_bounds__(SCAN_NT) k_scan_add(Job j) { scan_add_body(j, blockIdx.x); }
 __global__ void __launch_bounds__(SCAN_NT) k_scan_add_jobs(const Job* __restrict__ jobs) {
@@ -721,10 +710,7 @@ __device__ __forceinline__ void mass_finalize_body(const Job& j, int ntiles) {
   }
 }
 __global__ void __launch_bounds__(MASS_NT) k_mass_finalize(Job j, int ntiles) { mass_finalize_body(j, ntiles); }
-__global__ void __launch_bounds__(MASS_NT) k_mass_finalize_jobs(const Job* __restrict__ jobs) {
-  PK_JOB_OR_RETURN(jobs, 1);
-  mass_finalize_body(j, j.g.ntiles);
-}
+
 
 // ------------------------------------------------------------------------------------
 // host side
@@ -914,7 +900,6 @@ int scatter_run(Job& j, bool direct, const float* mesh_in, float* mesh_out, void
       hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_NT), 0, s, j);
     } else {
       hipLaunchKernelGGL(k_scan_local, dim3(nb2), dim3(SCAN_NT), 0, s, j);
-      hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_NT), 0, s, j);
       hipLaunchKernelGGL(k_scan_add, dim3(nb2), dim3(SCAN_NT), 0, s, j);
     }
     int e = check_hip(hipGetLastError(), "scan launch");
@@ -987,7 +972,6 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
     const int nb2 = (d.max_nkeys + SCAN_BLOCK - 1) / SCAN_BLOCK;
     hipLaunchKernelGGL(k_col_scan_jobs, dim3((d.max_nkeys + COL_KEYS - 1) / COL_KEYS, nj), dim3(COL_NT), 0, s, jobs);
     hipLaunchKernelGGL(k_scan_local_jobs, dim3(nb2, nj), dim3(SCAN_NT), 0, s, jobs);
-    hipLaunchKernelGGL(k_scan_top_jobs, dim3(1, nj), dim3(SCAN_NT), 0, s, jobs);
     hipLaunchKernelGGL(k_scan_add_jobs, dim3(nb2, nj), dim3(SCAN_NT), 0, s, jobs);
     int e = check_hip(hipGetLastError(), "batch scan launch");
     if (e) return e;
@@ -1008,10 +992,5 @@ int scatter_sort_jobs(const Job* jobs, const BatchDims& d, hipStream_t s) {
   return check_hip(hipGetLastError(), "batch fine place launch");
 }
 
-int mass_finalize_jobs(const Job* jobs, int njobs, hipStream_t s) {
-  StageTimer t(ST_MASS, s);
-  hipLaunchKernelGGL(k_mass_finalize_jobs, dim3(1, njobs), dim3(MASS_NT), 0, s, jobs);
-  return check_hip(hipGetLastError(), "batch mass finalize launch");
-}
 
 }  // namespace pk
