@@ -139,7 +139,7 @@ __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const 
       const int p = B.pre[b + step];   // b + step <= 15: inside pre[] even beyond nbk
       b = (b + step < B.nbk && p <= i) ? b + step : b;
     }
-    return load_record_raw(sorted, (size_t)B.base[b] + (i - B.pre[b]), has_w);
+    return load_record_stream(sorted, (size_t)B.base[b] + (i - B.pre[b]), has_w);
   };
   auto regular = [&](const TileBuckets& B, int idx) -> bool { return !irregular || idx < B.pre[B.nbk - 1]; };
   // Software pipeline over this block's tiles: the next tile's bucket bounds load during the

@@ -24,26 +24,33 @@ __device__ __forceinline__ int floor_mod(int a, int n) {
 // AOS / W: the input layout (x, y, z interleaved) and the presence of weights, fixed at
 // compile time (0 / 1) or read from a (-1).  Fixed values leave no branch between the loads
 // of a thread's particles, so they are all in flight together; the position arithmetic
-// follows the loads.
-template <int AOS = -1, int W = -1>
+// follows the loads.  STREAM: non-temporal loads, for the last read of the positions in an
+// estimate, so that they do not displace in the caches what the next kernels read.
+template <int AOS = -1, int W = -1, bool STREAM = false>
 __device__ __forceinline__ void load_particle(const ScatterArgs& a, int64_t i, float& px, float& py, float& pz,
                                               float& w) {
   const bool aos = AOS >= 0 ? AOS == 1 : a.aos != 0;
   const bool hw = W >= 0 ? W == 1 : a.w != nullptr;
+  auto ld = [](const float* p) -> float {
+    if constexpr (STREAM) return __builtin_nontemporal_load(p);
+    else return *p;
+  };
   float x, y, z;
   if (aos) {   // one 12-byte load per particle
     typedef float f3 __attribute__((ext_vector_type(3)));
+    typedef float f3a __attribute__((ext_vector_type(3), aligned(4)));
     f3 v;
-    __builtin_memcpy(&v, a.x + 3 * i, sizeof(float) * 3);
+    if constexpr (STREAM) v = __builtin_nontemporal_load(reinterpret_cast<const f3a*>(a.x + 3 * i));
+    else __builtin_memcpy(&v, a.x + 3 * i, sizeof(float) * 3);
     x = v.x;
     y = v.y;
     z = v.z;
   } else {
-    x = a.x[i * a.sx];
-    y = a.y[i * a.sy];
-    z = a.z[i * a.sz];
+    x = ld(a.x + i * a.sx);
+    y = ld(a.y + i * a.sy);
+    z = ld(a.z + i * a.sz);
   }
-  w = hw ? a.w[i * a.sw] : 1.0f;
+  w = hw ? ld(a.w + i * a.sw) : 1.0f;
   px = (x - a.xmin) * a.inv;
   py = (y - a.ymin) * a.inv;
   pz = (z - a.zmin) * a.inv;
@@ -304,6 +311,15 @@ __device__ __forceinline__ float4 load_record_raw(const void* sorted, size_t i, 
   typedef float f4 __attribute__((ext_vector_type(4), aligned(4)));
   const float* p = static_cast<const float*>(sorted) + i * (has_w ? 4 : 3);
   const f4 v = *reinterpret_cast<const f4*>(p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+// The same as a non-temporal load, for the last read of a record array in an estimate (the
+// staged records in the fine place, the sorted ones in the fused deposit): the 0.24 GB of
+// records then do not displace in the caches what the next kernels read.
+__device__ __forceinline__ float4 load_record_stream(const void* sorted, size_t i, bool has_w) {
+  typedef float f4 __attribute__((ext_vector_type(4), aligned(4)));
+  const float* p = static_cast<const float*>(sorted) + i * (has_w ? 4 : 3);
+  const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
   return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ float record_weight(const float4& r, bool has_w) { return has_w ? r.w : 1.0f; }

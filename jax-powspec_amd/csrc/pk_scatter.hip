@@ -414,7 +414,7 @@ __device__ __forceinline__ void bucket_place_body(const Job& j, int bx, unsigned
   auto load_round = [&](int64_t r0, float* x, float* y, float* z, float* ww) {
 #pragma unroll
     for (int u = 0; u < PLACE_PER; ++u)
-      load_particle<AOS, W>(a, min(r0 + (int64_t)u * CNT_NT + tid, hi - 1), x[u], y[u], z[u], ww[u]);
+      load_particle<AOS, W, true>(a, min(r0 + (int64_t)u * CNT_NT + tid, hi - 1), x[u], y[u], z[u], ww[u]);
   };
   if (lo >= hi) return;   // whole block: an empty chunk
   load_round(lo, px, py, pz, w);
@@ -542,7 +542,7 @@ __device__ __forceinline__ void fine_place_body(const Job& j, int bx) {
   int key[FINE_PER];
 #pragma unroll
   for (int u = 0; u < FINE_PER; ++u)   // clamped: no branch between the loads
-    r[u] = load_record_raw(j.staged, (size_t)min(s0 + tid + (int64_t)u * FINE_NT, s1 - 1), has_w);
+    r[u] = load_record_stream(j.staged, (size_t)min(s0 + tid + (int64_t)u * FINE_NT, s1 - 1), has_w);
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int u = 0; u < FINE_PER; ++u) {
