@@ -24,8 +24,9 @@ __device__ __forceinline__ int floor_mod(int a, int n) {
 // AOS / W: the input layout (x, y, z interleaved) and the presence of weights, fixed at
 // compile time (0 / 1) or read from a (-1).  Fixed values leave no branch between the loads
 // of a thread's particles, so they are all in flight together; the position arithmetic
-// follows the loads.  STREAM: non-temporal loads, for the last read of the positions in an
-// estimate, so that they do not displace in the caches what the next kernels read.
+// follows the loads.  STREAM: non-temporal loads, for the two reads of the positions in an
+// estimate (count, coarse place), so that they do not displace in the caches what the next
+// kernels read (keeping them cached between the two reads measured slower overall).
 template <int AOS = -1, int W = -1, bool STREAM = false>
 __device__ __forceinline__ void load_particle(const ScatterArgs& a, int64_t i, float& px, float& py, float& pz,
                                               float& w) {

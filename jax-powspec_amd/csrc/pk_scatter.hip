@@ -100,7 +100,7 @@ __device__ __forceinline__ void bucket_count_body(const Job& j, int bx, CountSha
     // last particle and are not counted)
 #pragma unroll
     for (int u = 0; u < CNT_UNROLL; ++u)
-      load_particle<AOS, W>(a, min(i0 + (int64_t)u * CNT_NT, hi - 1), px[u], py[u], pz[u], w[u]);
+      load_particle<AOS, W, true>(a, min(i0 + (int64_t)u * CNT_NT, hi - 1), px[u], py[u], pz[u], w[u]);
     __builtin_amdgcn_sched_barrier(0);   // every load issued before the first key
 #pragma unroll
     for (int u = 0; u < CNT_UNROLL; ++u) {
