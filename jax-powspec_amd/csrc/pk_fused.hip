@@ -69,7 +69,7 @@ template <int N, int TY, int ST>
 __device__ __forceinline__ void fused_body(const Job& j, int bx, int nbx, const float2* __restrict__ twg) {
   using Cf = FusedCfg<N, TY>;
   constexpr int NC = Cf::NC, CS = Cf::CS, C = Cf::TY, NT = Cf::NT;
-  constexpr int U = 2;   // records per thread loaded ahead
+  constexpr int U = 3;   // records per thread loaded ahead (2 and 4 measured slower)
   extern __shared__ float2 buf[];   // N x C int64 cells (z-major), then C rows of CS complex
   __shared__ float2 tw[N];   // full table: the split and the stages read one entry each
   __shared__ TileBuckets tb[2];
